@@ -1,0 +1,299 @@
+// vbg_api.cpp -- the C ABI (include/vbgemm.h): plan objects, device work-map
+// upload, launches and error reporting.  Errors are returned as vbg_status
+// codes; nothing here exits the process (the reference's ErrChk prints and
+// exits, ROCm/include/util.h:58-64).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vbgemm.h"
+#include "vbg_launch.h"
+#include "vbg_planner.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return VBG_ERR_HIP;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int occupancy_cb(int family, int tile, bool ta, bool tb) {
+  switch (family) {
+    case vbg::FAM_FP32: return vbg_occupancy_fp32(tile, ta, tb);
+    case vbg::FAM_SEQ: return vbg_occupancy_seq(ta, tb);
+    case vbg::FAM_SPLIT: return vbg_occupancy_split(tile, ta, tb);
+    default: return 1;
+  }
+}
+
+}  // namespace
+
+struct vbg_plan {
+  vbg::HostPlan hp;
+  vbg_opts opts;
+  int device = -1;
+  bool host_only = false;
+  void* dbuf = nullptr;  // one device allocation: descs, items, queues, counters, scales
+  vbg_mat_desc* d_desc = nullptr;
+  float* d_scales = nullptr;  // [2 * batch] (FP16 / FP8 paths)
+  unsigned* d_amax = nullptr; // [2 * batch] float bits of per-matrix amax
+  struct DevLaunch {
+    vbg_item* items;
+    int* q_begin;
+    unsigned* counters;
+  };
+  std::vector<DevLaunch> dl;
+};
+
+extern "C" {
+
+int vbg_abi_version(void) { return VBG_ABI_VERSION; }
+
+const char* vbg_status_string(int s) {
+  switch (s) {
+    case VBG_OK: return "VBG_OK";
+    case VBG_ERR_INVALID_ARG: return "VBG_ERR_INVALID_ARG";
+    case VBG_ERR_HIP: return "VBG_ERR_HIP";
+    case VBG_ERR_NOT_SUPPORTED: return "VBG_ERR_NOT_SUPPORTED";
+    case VBG_ERR_ALLOC: return "VBG_ERR_ALLOC";
+    case VBG_ERR_PLAN_MISMATCH: return "VBG_ERR_PLAN_MISMATCH";
+    case VBG_ERR_UNDEFINED: return "VBG_ERR_UNDEFINED";
+    default: return "VBG_ERR_UNKNOWN";
+  }
+}
+
+const char* vbg_last_error(void) { return g_err.c_str(); }
+
+int vbg_opts_default(vbg_opts* o) {
+  if (!o) return fail(VBG_ERR_INVALID_ARG, "null opts");
+  std::memset(o, 0, sizeof(*o));
+  o->policy = VBG_POLICY_AUTO;
+  o->wavefront_thres = 6000;  // ROCm/framework/run.sh:4
+  o->tlp_thres = 6000;
+  o->fixed_tile = VBG_TILE_128x128;
+  o->xcd_queues = 1;
+  return VBG_OK;
+}
+
+int vbg_ref_plan(int batch, const int* M, const int* N, const int* K, int wavefront_thres,
+                 int* t_strategy, int* kernel_thread, int* grid, int64_t* wavefronts) {
+  int rc = vbg::ref_plan(batch, M, N, K, wavefront_thres, t_strategy, kernel_thread, grid, wavefronts);
+  if (rc == VBG_ERR_UNDEFINED) g_err = "reference planner undefined for this batch";
+  else if (rc != VBG_OK) g_err = "invalid argument";
+  return rc;
+}
+
+int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, const int* K,
+                    const int* lda, const int* ldb, const int* ldc, char transA, char transB,
+                    int prec, const vbg_opts* opts) {
+  if (!plan) return fail(VBG_ERR_INVALID_ARG, "null plan out-pointer");
+  *plan = nullptr;
+  auto tr = [](char c, bool& t) {
+    if (c == 'N' || c == 'n') t = false;
+    else if (c == 'T' || c == 't' || c == 'C' || c == 'c') t = true;
+    else return false;
+    return true;
+  };
+  bool ta, tb;
+  if (!tr(transA, ta) || !tr(transB, tb)) return fail(VBG_ERR_INVALID_ARG, "trans must be 'N' or 'T'");
+  vbg_opts o;
+  vbg_opts_default(&o);
+  if (opts) o = *opts;
+  const bool host_only = o.host_only != 0;
+
+  int dev = -1, cus = o.num_cus > 0 ? o.num_cus : 256;
+  if (!host_only) {
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    cus = prop.multiProcessorCount;
+  }
+  vbg_plan* p = new (std::nothrow) vbg_plan();
+  if (!p) return fail(VBG_ERR_ALLOC, "host allocation");
+  p->opts = o;
+  p->device = dev;
+  p->host_only = host_only;
+  std::string err;
+  int rc = vbg::build_plan(p->hp, batch, M, N, K, lda, ldb, ldc, ta, tb, prec, o, cus,
+                           host_only ? nullptr : occupancy_cb, err);
+  if (rc != VBG_OK) {
+    delete p;
+    return fail(rc, err);
+  }
+  if (!host_only) {
+    const vbg::HostPlan& hp = p->hp;
+    // layout
+    size_t off = 0;
+    const size_t off_desc = off;
+    off = align256(off + sizeof(vbg_mat_desc) * std::max(1, hp.batch));
+    const size_t off_scales = off;
+    off = align256(off + sizeof(float) * 2 * std::max(1, hp.batch));
+    const size_t off_amax = off;
+    off = align256(off + sizeof(unsigned) * 2 * std::max(1, hp.batch));
+    std::vector<size_t> oi, oq, oc;
+    for (const auto& L : hp.launches) {
+      oi.push_back(off);
+      off = align256(off + sizeof(vbg_item) * L.items.size());
+      oq.push_back(off);
+      off = align256(off + sizeof(int) * L.q_begin.size());
+      oc.push_back(off);
+      off = align256(off + sizeof(unsigned) * (hp.nq + 1));
+    }
+    std::vector<char> host(off, 0);
+    if (hp.batch) std::memcpy(host.data() + off_desc, hp.desc.data(), sizeof(vbg_mat_desc) * hp.batch);
+    for (size_t l = 0; l < hp.launches.size(); ++l) {
+      const auto& L = hp.launches[l];
+      std::memcpy(host.data() + oi[l], L.items.data(), sizeof(vbg_item) * L.items.size());
+      std::memcpy(host.data() + oq[l], L.q_begin.data(), sizeof(int) * L.q_begin.size());
+    }
+    hipError_t e = hipMalloc(&p->dbuf, off);
+    if (e != hipSuccess) {
+      delete p;
+      return hip_fail(e, "hipMalloc(plan)");
+    }
+    e = hipMemcpy(p->dbuf, host.data(), off, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(p->dbuf);
+      delete p;
+      return hip_fail(e, "hipMemcpy(plan)");
+    }
+    char* base = (char*)p->dbuf;
+    p->d_desc = (vbg_mat_desc*)(base + off_desc);
+    p->d_scales = (float*)(base + off_scales);
+    p->d_amax = (unsigned*)(base + off_amax);
+    for (size_t l = 0; l < hp.launches.size(); ++l)
+      p->dl.push_back({(vbg_item*)(base + oi[l]), (int*)(base + oq[l]), (unsigned*)(base + oc[l])});
+  }
+  *plan = p;
+  return VBG_OK;
+}
+
+int vbg_plan_destroy(vbg_plan* p) {
+  if (!p) return VBG_OK;
+  int rc = VBG_OK;
+  if (p->dbuf) {
+    hipError_t e = hipFree(p->dbuf);
+    if (e != hipSuccess) rc = hip_fail(e, "hipFree(plan)");
+  }
+  delete p;
+  return rc;
+}
+
+int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const float* const* dB,
+                       float beta, float* const* dC, void* stream) {
+  if (!p) return fail(VBG_ERR_INVALID_ARG, "null plan");
+  if (p->host_only) return fail(VBG_ERR_PLAN_MISMATCH, "host_only plan cannot run");
+  const vbg::HostPlan& hp = p->hp;
+  if (hp.batch == 0) return VBG_OK;
+  if (!dA || !dB || !dC) return fail(VBG_ERR_INVALID_ARG, "null pointer array");
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev != p->device) return fail(VBG_ERR_PLAN_MISMATCH, "plan was created on another device");
+  hipStream_t s = (hipStream_t)stream;
+
+  if (hp.prec == VBG_PREC_FP16_SCALED || hp.prec == VBG_PREC_FP8_E4M3) {
+    int r = vbg_launch_scales(hp.prec, hp.batch, hp.ta, hp.tb, p->d_desc, dA, dB, p->d_amax,
+                              p->d_scales, s);
+    if (r) return fail(VBG_ERR_HIP, std::string("scale pre-pass launch: ") + hipGetErrorString(hipGetLastError()));
+  }
+  for (size_t l = 0; l < hp.launches.size(); ++l) {
+    const vbg::Launch& L = hp.launches[l];
+    vbg_args a;
+    a.desc = p->d_desc;
+    a.items = p->dl[l].items;
+    a.q_begin = p->dl[l].q_begin;
+    a.counters = p->dl[l].counters;
+    a.A = dA;
+    a.B = dB;
+    a.C = dC;
+    a.ws = nullptr;
+    a.ws_off = nullptr;
+    a.scales = p->d_scales;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.nq = hp.nq;
+    a.trans_a = hp.ta;
+    a.trans_b = hp.tb;
+    a.prec = hp.prec;
+    int r;
+    switch (L.family) {
+      case vbg::FAM_FP32: r = vbg_launch_fp32(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
+      case vbg::FAM_SEQ: r = vbg_launch_seq(hp.ta, hp.tb, &a, L.grid, s); break;
+      case vbg::FAM_SPLIT: r = vbg_launch_split(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
+      default: r = -1;
+    }
+    if (r == -1) return fail(VBG_ERR_NOT_SUPPORTED, "tile/precision combination not built");
+    if (r != 0) return fail(VBG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(hipGetLastError()));
+  }
+  return VBG_OK;
+}
+
+int vbg_sgemm_vbatched_oneshot(int batch, const int* M, const int* N, const int* K, const int* lda,
+                               const int* ldb, const int* ldc, char transA, char transB, int prec,
+                               float alpha, const float* const* dA, const float* const* dB,
+                               float beta, float* const* dC, void* stream) {
+  vbg_plan* p = nullptr;
+  int rc = vbg_plan_create(&p, batch, M, N, K, lda, ldb, ldc, transA, transB, prec, nullptr);
+  if (rc != VBG_OK) return rc;
+  rc = vbg_sgemm_vbatched(p, alpha, dA, dB, beta, dC, stream);
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  if (rc == VBG_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+  int rc2 = vbg_plan_destroy(p);
+  return rc != VBG_OK ? rc : rc2;
+}
+
+int vbg_plan_query(const vbg_plan* p, int what, int* out, int out_len) {
+  if (!p) return fail(VBG_ERR_INVALID_ARG, "null plan");
+  const vbg::HostPlan& hp = p->hp;
+  int64_t n = 0;
+  for (const auto& L : hp.launches) n += (int64_t)L.items.size();
+  switch (what) {
+    case 0: return (int)n;
+    case 1: return (int)hp.launches.size();
+    case 3: return hp.launches.empty() ? 0 : hp.launches[0].grid;
+    case 2:
+    case 5: {
+      if (!out || out_len < hp.batch) return fail(VBG_ERR_INVALID_ARG, "out too small");
+      for (int b = 0; b < hp.batch; ++b) out[b] = what == 2 ? hp.tile_of[b] : hp.desc[b].ksplit;
+      return hp.batch;
+    }
+    case 4: {
+      if (!out || out_len < 5 * n) return fail(VBG_ERR_INVALID_ARG, "out too small");
+      int w = 0;
+      for (const auto& L : hp.launches)
+        for (const auto& it : L.items) {
+          const vbg_mat_desc& d = hp.desc[it.mat];
+          int kb = 0, ke = d.K;
+          if (d.ksplit > 1) {
+            const int bk = 32;
+            const int kt = (d.K + bk - 1) / bk, per = (kt + d.ksplit - 1) / d.ksplit;
+            kb = std::min(d.K, it.kslice * per * bk);
+            ke = std::min(d.K, kb + per * bk);
+          }
+          out[w++] = it.mat;
+          out[w++] = it.tm;
+          out[w++] = it.tn;
+          out[w++] = kb;
+          out[w++] = ke;
+        }
+      return w;
+    }
+    default: return fail(VBG_ERR_INVALID_ARG, "unknown query");
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,377 @@
+// vbg_fp32.hip -- FP32 tile kernels for gfx950.
+//
+//  * Fp32MfmaTile<BM, BN>: 256 threads = 2x2 waves, each wave owns a
+//    (BM/2) x (BN/2) sub-tile built from 32x32 blocks of
+//    v_mfma_f32_32x32x2_f32.  Operands stream HBM -> registers -> LDS
+//    (double-buffered, one barrier per k-tile); the next k-tile's HBM loads are
+//    in flight while the MFMAs consume the current LDS buffer.
+//  * SeqTile: the VALU reference-compatible tile, one fmaf chain per output
+//    element over ascending k.
+//
+// Replaces the reference's tile functions gemm_128_16x16 ... gemm_256_64x64
+// (ROCm/framework/kernel_128_new.h, kernel_256_new.h) and its dispatcher
+// (ROCm/framework/kernel.h:9-42): a work item names the matrix and tile, so no
+// workgroup is launched for an empty bounding-box slot.
+#include "vbg_tiles.h"
+
+namespace {
+
+// MFMA k-slot layout used by every FP32 MFMA tile: v_mfma_f32_32x32x2_f32
+// sums over two k-slots, lanes 0-31 supplying slot 0 and lanes 32-63 slot 1.
+// Any assignment of actual k to (slot, step) works as long as A and B agree;
+// lane half h takes k in [h*BK/2, (h+1)*BK/2), so each lane reads its k values
+// as contiguous float4s from the [mn][k] LDS layout.
+template <int BM, int BN, int BK, bool TA, bool TB>
+struct Fp32MfmaTile {
+  static constexpr int P = BK + 4;  // LDS pitch (floats); +4 staggers rows across banks
+  static constexpr int STAGE = (BM + BN) * P;
+  static constexpr int LDS_FLOATS = 2 * STAGE;
+  static constexpr int WTM = BM / 2, WTN = BN / 2;
+  static constexpr int TM = WTM / 32, TN = WTN / 32;
+  static_assert(TM >= 1 && TN >= 1, "FP32 MFMA tile needs >= 64x64");
+
+  // A tile: op(A) rows = M index.  op N: A[i + k*lda] (mn contiguous);
+  // op T: A[k + i*lda] (k contiguous).  B tile: op(B) columns = N index.
+  // op N: B[k + j*ldb] (k contiguous); op T: B[j + k*ldb] (mn contiguous).
+  using LA = TileLoader<BM, BK, P, TA>;
+  using LB = TileLoader<BN, BK, P, !TB>;
+
+  __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
+    const vbg_mat_desc d = a.desc[it.mat];
+    const float* __restrict__ A = a.A[it.mat];
+    const float* __restrict__ B = a.B[it.mat];
+    float* __restrict__ C = a.C[it.mat];
+    const int m0 = it.tm * BM, n0 = it.tn * BN;
+    int kb, ke;
+    vbg_k_range<BK>(d.K, d.ksplit, it.kslice, kb, ke);
+    const int nk = (ke - kb + BK - 1) / BK;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
+    const int l32 = lane & 31, h = lane >> 5;
+
+    v16f acc[TM][TN];
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int s = 0; s < TN; ++s)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[r][s][e] = 0.0f;
+
+    LA la;
+    LB lb;
+    if (nk > 0) {
+      const bool tail0 = kb + BK > ke;
+      la.load(A, d.lda, m0, kb, d.M, ke, tail0);
+      lb.load(B, d.ldb, n0, kb, d.N, ke, tail0);
+      la.store(lds);
+      lb.store(lds + BM * P);
+    }
+    __syncthreads();
+
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) {
+        const int k0 = kb + (kt + 1) * BK;
+        const bool tail = k0 + BK > ke;
+        la.load(A, d.lda, m0, k0, d.M, ke, tail);
+        lb.load(B, d.ldb, n0, k0, d.N, ke, tail);
+      }
+      const float* As = lds + cur * STAGE;
+      const float* Bs = As + BM * P;
+#pragma unroll
+      for (int c = 0; c < BK / 8; ++c) {
+        v4f av[TM], bv[TN];
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+          av[r] = *(const v4f*)(As + (wr * WTM + r * 32 + l32) * P + h * (BK / 2) + 4 * c);
+#pragma unroll
+        for (int s = 0; s < TN; ++s)
+          bv[s] = *(const v4f*)(Bs + (wc * WTN + s * 32 + l32) * P + h * (BK / 2) + 4 * c);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int r = 0; r < TM; ++r)
+#pragma unroll
+            for (int s = 0; s < TN; ++s)
+              acc[r][s] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[r][e], bv[s][e], acc[r][s], 0, 0, 0);
+      }
+      if (more) {
+        float* nxt = lds + (cur ^ 1) * STAGE;
+        la.store(nxt);
+        lb.store(nxt + BM * P);
+      }
+      __syncthreads();
+    }
+
+    // 32x32x2 f32 accumulator layout: register 4g+e of lane l holds
+    // row 8g + 4*(l/32) + e, column l%32 of the 32x32 block.
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int s = 0; s < TN; ++s)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
+          const int col = n0 + wc * WTN + s * 32 + l32;
+          v4f v = {acc[r][s][4 * g], acc[r][s][4 * g + 1], acc[r][s][4 * g + 2], acc[r][s][4 * g + 3]};
+          vbg_store4(C, d.ldc, row, col, d.M, d.N, v, a.alpha, a.beta);
+        }
+  }
+};
+
+// Small FP32 MFMA tiles on v_mfma_f32_16x16x4_f32 (16x16 block, 4 k-slots:
+// lane l supplies A[l%16][slot l/16] and B[slot l/16][l%16]; register r of
+// lane l holds D[4*(l/16) + r][l%16]).
+//   BMN = 32: 2x2 waves, one 16x16 block each; lane group g = l/16 takes
+//             k in [g*BK/4, (g+1)*BK/4) of every k-tile.
+//   BMN = 16: the four waves split each k-tile four ways (wave w takes
+//             [w*BK/4, (w+1)*BK/4), lane group g a quarter of that) and the
+//             partial 16x16 blocks are summed through LDS at the end -- the
+//             wavefront-level reduction for tiles too small to occupy 4 waves.
+template <int BMN, bool TA, bool TB>
+struct Fp32SmallTile {
+  static constexpr bool KSPLIT = BMN == 16;
+  static constexpr int BK = KSPLIT ? 64 : 32;
+  static constexpr int P = BK + 4;
+  static constexpr int STAGE = 2 * BMN * P;
+  static constexpr int LDS_FLOATS = 2 * STAGE;
+  static constexpr int KL = KSPLIT ? BK / 16 : BK / 4;  // k per lane per k-tile
+  static_assert(KL % 4 == 0, "lane k range must be whole float4s");
+  static_assert(!KSPLIT || LDS_FLOATS >= 4 * 256, "reduction buffer must fit in LDS");
+  using LA = TileLoader<BMN, BK, P, TA>;
+  using LB = TileLoader<BMN, BK, P, !TB>;
+
+  __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
+    const vbg_mat_desc d = a.desc[it.mat];
+    const float* __restrict__ A = a.A[it.mat];
+    const float* __restrict__ B = a.B[it.mat];
+    float* __restrict__ C = a.C[it.mat];
+    const int m0 = it.tm * BMN, n0 = it.tn * BMN;
+    int kb, ke;
+    vbg_k_range<BK>(d.K, d.ksplit, it.kslice, kb, ke);
+    const int nk = (ke - kb + BK - 1) / BK;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
+    const int arow = KSPLIT ? l16 : (wave & 1) * 16 + l16;
+    const int bcol = KSPLIT ? l16 : (wave >> 1) * 16 + l16;
+    const int koff = KSPLIT ? wave * (BK / 4) + g * KL : g * KL;
+
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    LA la;
+    LB lb;
+    if (nk > 0) {
+      const bool tail0 = kb + BK > ke;
+      la.load(A, d.lda, m0, kb, d.M, ke, tail0);
+      lb.load(B, d.ldb, n0, kb, d.N, ke, tail0);
+      la.store(lds);
+      lb.store(lds + BMN * P);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) {
+        const int k0 = kb + (kt + 1) * BK;
+        const bool tail = k0 + BK > ke;
+        la.load(A, d.lda, m0, k0, d.M, ke, tail);
+        lb.load(B, d.ldb, n0, k0, d.N, ke, tail);
+      }
+      const float* As = lds + cur * STAGE;
+      const float* Bs = As + BMN * P;
+#pragma unroll
+      for (int c = 0; c < KL / 4; ++c) {
+        const v4f av = *(const v4f*)(As + arow * P + koff + 4 * c);
+        const v4f bv = *(const v4f*)(Bs + bcol * P + koff + 4 * c);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], bv[e], acc, 0, 0, 0);
+      }
+      if (more) {
+        float* nxt = lds + (cur ^ 1) * STAGE;
+        la.store(nxt);
+        lb.store(nxt + BMN * P);
+      }
+      __syncthreads();
+    }
+    const int row0 = 4 * g;  // rows row0..row0+3 of the 16x16 block, column l16
+    if (KSPLIT) {
+      *(v4f*)(lds + wave * 256 + lane * 4) = acc;
+      __syncthreads();
+      if (wave == 0) {
+        v4f s = acc;
+#pragma unroll
+        for (int w = 1; w < 4; ++w) s += *(const v4f*)(lds + w * 256 + lane * 4);
+        vbg_store4(C, d.ldc, m0 + row0, n0 + l16, d.M, d.N, s, a.alpha, a.beta);
+      }
+      __syncthreads();
+    } else {
+      vbg_store4(C, d.ldc, m0 + (wave & 1) * 16 + row0, n0 + (wave >> 1) * 16 + l16, d.M, d.N, acc,
+                 a.alpha, a.beta);
+    }
+  }
+};
+
+// Reference-compatible VALU tile (32x32 outputs, 4 per thread).  Each output
+// is one fmaf chain over k = 0, 1, ..., K-1 in order.  With alpha = beta = 1
+// the chain is seeded from C, which is exactly the reference's correct tiles
+// (ROCm/framework/kernel_128_new.h:160-187, kernel_256_new.h:233-260);
+// otherwise it starts from 0 and C = alpha*acc + beta*C.
+template <bool TA, bool TB>
+struct SeqTile {
+  static constexpr int BM = 32, BN = 32, BK = 32;
+  static constexpr int PA = 36, PB = 33;  // As[k][i] (float4 reads), Bs[k][j]
+  static constexpr int LDS_FLOATS = BK * PA + BK * PB;
+
+  __device__ static float ga(const float* A, int lda, int i, int k) {
+    return TA ? A[(size_t)i * lda + k] : A[i + (size_t)k * lda];
+  }
+  __device__ static float gb(const float* B, int ldb, int k, int j) {
+    return TB ? B[j + (size_t)k * ldb] : B[(size_t)j * ldb + k];
+  }
+
+  __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
+    const vbg_mat_desc d = a.desc[it.mat];
+    const float* A = a.A[it.mat];
+    const float* B = a.B[it.mat];
+    float* C = a.C[it.mat];
+    const int m0 = it.tm * BM, n0 = it.tn * BN;
+    float* As = lds;
+    float* Bs = lds + BK * PA;
+    const int t = threadIdx.x;
+    const int jj = t >> 3, ii = 4 * (t & 7);
+    const int col = n0 + jj, row = m0 + ii;
+    const bool seeded = (a.alpha == 1.0f && a.beta == 1.0f);
+    float c[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      c[e] = 0.0f;
+      if (seeded && col < d.N && row + e < d.M) c[e] = C[(size_t)col * d.ldc + row + e];
+    }
+    for (int k0 = 0; k0 < d.K; k0 += BK) {
+      const int kn = min(BK, d.K - k0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int idx = t + VBG_THREADS * v;
+        {  // A: i fastest for coalescing when op(A) = A
+          const int i = idx & 31, k = idx >> 5;
+          const int gi = m0 + i, gk = k0 + k;
+          As[k * PA + i] = (gi < d.M && gk < d.K) ? ga(A, d.lda, gi, gk) : 0.0f;
+        }
+        {  // B: k fastest for coalescing when op(B) = B
+          const int k = idx & 31, j = idx >> 5;
+          const int gk = k0 + k, gj = n0 + j;
+          Bs[k * PB + j] = (gj < d.N && gk < d.K) ? gb(B, d.ldb, gk, gj) : 0.0f;
+        }
+      }
+      __syncthreads();
+      for (int k = 0; k < kn; ++k) {
+        const v4f av = *(const v4f*)(As + k * PA + ii);
+        const float bv = Bs[k * PB + jj];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c[e] = fmaf(av[e], bv, c[e]);
+      }
+      __syncthreads();
+    }
+    if (col < d.N) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (row + e < d.M) {
+          float* p = C + (size_t)col * d.ldc + row + e;
+          if (seeded) {
+            *p = c[e];
+          } else {
+            float o = a.alpha * c[e];
+            if (a.beta != 0.0f) o += a.beta * *p;
+            *p = o;
+          }
+        }
+      }
+    }
+  }
+};
+
+template <class Tile>
+__global__ void __launch_bounds__(VBG_THREADS) vbg_gemm_kernel(vbg_args a) {
+  __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
+  vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host-side launch table.
+#include "vbg_launch.h"
+
+template <class Tile>
+static hipError_t launch_one(const vbg_args& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(vbg_gemm_kernel<Tile>, dim3(grid), dim3(VBG_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class Tile>
+static int occupancy_one() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, vbg_gemm_kernel<Tile>, VBG_THREADS, 0) !=
+      hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
+}
+
+template <class Tile>
+static int run_or_occ(const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  if (occ) {
+    *occ = occupancy_one<Tile>();
+    return 0;
+  }
+  return launch_one<Tile>(*a, grid, s) == hipSuccess ? 0 : -2;
+}
+
+template <template <bool, bool> class TileTT>
+static int dispatch_tt(bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  if (!ta && !tb) return run_or_occ<TileTT<false, false>>(a, grid, s, occ);
+  if (!ta && tb) return run_or_occ<TileTT<false, true>>(a, grid, s, occ);
+  if (ta && !tb) return run_or_occ<TileTT<true, false>>(a, grid, s, occ);
+  return run_or_occ<TileTT<true, true>>(a, grid, s, occ);
+}
+
+template <bool TA, bool TB> using F32T16 = Fp32SmallTile<16, TA, TB>;
+template <bool TA, bool TB> using F32T32 = Fp32SmallTile<32, TA, TB>;
+template <bool TA, bool TB> using F32T64 = Fp32MfmaTile<64, 64, 32, TA, TB>;
+template <bool TA, bool TB> using F32T128x64 = Fp32MfmaTile<128, 64, 32, TA, TB>;
+template <bool TA, bool TB> using F32T64x128 = Fp32MfmaTile<64, 128, 32, TA, TB>;
+template <bool TA, bool TB> using F32T128 = Fp32MfmaTile<128, 128, 32, TA, TB>;
+
+static int fp32_dispatch(int tile, bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s,
+                         int* occ) {
+  switch (tile) {
+    case T16: return dispatch_tt<F32T16>(ta, tb, a, grid, s, occ);
+    case T32: return dispatch_tt<F32T32>(ta, tb, a, grid, s, occ);
+    case T64: return dispatch_tt<F32T64>(ta, tb, a, grid, s, occ);
+    case T128x64: return dispatch_tt<F32T128x64>(ta, tb, a, grid, s, occ);
+    case T64x128: return dispatch_tt<F32T64x128>(ta, tb, a, grid, s, occ);
+    case T128: return dispatch_tt<F32T128>(ta, tb, a, grid, s, occ);
+    default: return -1;
+  }
+}
+
+static int seq_dispatch(bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  return dispatch_tt<SeqTile>(ta, tb, a, grid, s, occ);
+}
+
+int vbg_launch_fp32(int tile, int ta, int tb, const vbg_args* a, int grid, void* stream) {
+  return fp32_dispatch(tile, ta != 0, tb != 0, a, grid, (hipStream_t)stream, nullptr);
+}
+int vbg_occupancy_fp32(int tile, int ta, int tb) {
+  int occ = 1;
+  fp32_dispatch(tile, ta != 0, tb != 0, nullptr, 0, nullptr, &occ);
+  return occ;
+}
+int vbg_launch_seq(int ta, int tb, const vbg_args* a, int grid, void* stream) {
+  return seq_dispatch(ta != 0, tb != 0, a, grid, (hipStream_t)stream, nullptr);
+}
+int vbg_occupancy_seq(int ta, int tb) {
+  int occ = 1;
+  seq_dispatch(ta != 0, tb != 0, nullptr, 0, nullptr, &occ);
+  return occ;
+}

@@ -1,0 +1,46 @@
+// vbg_internal.h -- structures shared by the host planner and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+// Per-matrix descriptor, uploaded once per plan (replaces the reference's
+// separate dev_M / dev_N / dev_K arrays, ROCm/framework/gemm.cuh:98-105).
+struct vbg_mat_desc {
+  int M, N, K;
+  int lda, ldb, ldc;
+  int tiles_n;  // tiles along N for the matrix's tile shape
+  int ksplit;   // number of K slices (1 = no split)
+};
+
+// One unit of work: a (matrix, tile_m, tile_n, k-slice) quadruple.
+struct vbg_item {
+  int mat;
+  int tm, tn;
+  int kslice;
+};
+
+// Arguments of every persistent GEMM kernel (passed by value).
+struct vbg_args {
+  const vbg_mat_desc* desc;
+  const vbg_item* items;
+  const int* q_begin;     // [nq + 1] item ranges of the work queues
+  unsigned* counters;     // [nq] queue heads, [nq] done counter
+  const float* const* A;  // device arrays of device pointers
+  const float* const* B;
+  float* const* C;
+  float* ws;              // split-K partial sums (ws_off per item slice), or null
+  const int64_t* ws_off;  // per-matrix offset into ws (floats), or null
+  const float* scales;    // per-matrix quantisation scales [2*batch] (FP16 / FP8), or null
+  float alpha, beta;
+  int nq;
+  int trans_a, trans_b;   // 0 = N, 1 = T
+  int prec;               // vbg_prec
+};
+
+// Tile ids (mirror vbg_tile in include/vbgemm.h).
+enum { T16 = 0, T32 = 1, T64 = 2, T128x64 = 3, T64x128 = 4, T128 = 5, NTILES = 6 };
+static const int kTileBM[NTILES] = {16, 32, 64, 128, 64, 128};
+static const int kTileBN[NTILES] = {16, 32, 64, 64, 128, 128};
+
+// Work-queue count of the XCD-aware scheduler: MI355X has 8 XCDs and
+// dispatches consecutive workgroup ids round-robin across them.
+#define VBG_NUM_XCD 8

@@ -1,0 +1,18 @@
+// vbg_launch.h -- host entry points of the kernel translation units, called by
+// the C ABI layer (vbg_api.cpp).  Return 0 on success, -1 for an unsupported
+// tile, -2 for a HIP launch error.
+#pragma once
+#include "vbg_internal.h"
+
+int vbg_launch_fp32(int tile, int ta, int tb, const vbg_args* a, int grid, void* stream);
+int vbg_occupancy_fp32(int tile, int ta, int tb);
+int vbg_launch_seq(int ta, int tb, const vbg_args* a, int grid, void* stream);
+int vbg_occupancy_seq(int ta, int tb);
+
+// Split-precision family (bf16x3, FP16 scaled, FP8 e4m3).
+int vbg_launch_split(int tile, int ta, int tb, const vbg_args* a, int grid, void* stream);
+int vbg_occupancy_split(int tile, int ta, int tb);
+// Per-matrix amax pre-pass and scale computation for FP16 / FP8.
+int vbg_launch_scales(int prec, int batch, int ta, int tb, const vbg_mat_desc* desc,
+                      const float* const* A, const float* const* B, unsigned* amax, float* scales,
+                      void* stream);

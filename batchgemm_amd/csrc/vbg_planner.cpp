@@ -1,0 +1,363 @@
+// vbg_planner.cpp -- host planner (no HIP calls).
+//
+// Replaces the reference planner (ROCm/framework/gemm.cuh:121-229) whose
+// output is a tile index per matrix plus a bounding-box grid in which ~98% of
+// the workgroups exit at once (grid (32,16,256) with 2,498 useful blocks on
+// the shipped dataset at B=256).  Here the planner emits an explicit work map:
+//   1. a tile shape per matrix from a cost model (or one of the reference /
+//      Wang / Li policies, restated for comparison and autotuning);
+//   2. one work item per (matrix, tile_m, tile_n) -- never an empty slot;
+//   3. matrices dealt to 8 per-XCD queues by longest-processing-time-first,
+//      each queue ordered by decreasing per-item cost, so that a persistent
+//      workgroup pops long items first and the tiles of one matrix share the
+//      L2 of one XCD.
+#include "vbg_planner.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "../../include/vbgemm.h"
+
+namespace vbg {
+
+namespace {
+
+// k-tile depth of each tile id (matches the kernels).
+const int kTileBK[NTILES] = {64, 32, 32, 32, 32, 32};
+
+// Relative MFMA efficiency of each tile shape on gfx950 (per-flop cost factor
+// = 1/eff).  Larger tiles reuse each LDS byte and each HBM byte more times.
+const double kTileEff[NTILES] = {0.20, 0.40, 0.70, 0.85, 0.85, 1.00};
+// Fixed cost of one work item in units of a 128x128x32 k-step (queue pop,
+// pipeline fill, epilogue).
+const double kItemFixed = 3.0;
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+struct Shape {
+  int M, N, K;
+};
+
+double item_cost(int tile, const Shape& s) {
+  const int bm = kTileBM[tile], bn = kTileBN[tile], bk = kTileBK[tile];
+  const double ksteps = (double)cdiv(std::max(s.K, 1), bk);
+  const double unit = 128.0 * 128.0 * 32.0;
+  return kItemFixed + ksteps * (double)bm * bn * bk / unit / kTileEff[tile];
+}
+
+int num_tiles(int tile, const Shape& s) {
+  return cdiv(s.M, kTileBM[tile]) * cdiv(s.N, kTileBN[tile]);
+}
+
+double matrix_cost(int tile, const Shape& s) { return num_tiles(tile, s) * item_cost(tile, s); }
+
+bool tile_allowed(int prec, int tile, unsigned mask) {
+  if (mask && !(mask & (1u << tile))) return false;
+  (void)prec;
+  return true;
+}
+
+// AUTO: per-matrix minimum modelled cost, then, while the batch exposes fewer
+// work items than resident workgroup slots, split the matrix whose single
+// item is longest into the next smaller tile (more, shorter items).
+void choose_auto(const std::vector<Shape>& sh, int prec, const vbg_opts& o, int slots,
+                 std::vector<int>& tile) {
+  const int B = (int)sh.size();
+  const int big_min = o.big_tile_min_mn > 0 ? o.big_tile_min_mn : 0;
+  for (int b = 0; b < B; ++b) {
+    int best = -1;
+    double bc = 0;
+    for (int t = 0; t < NTILES; ++t) {
+      if (!tile_allowed(prec, t, o.tile_mask)) continue;
+      if (t == T128 && big_min && std::min(sh[b].M, sh[b].N) < big_min) continue;
+      const double c = matrix_cost(t, sh[b]);
+      if (best < 0 || c < bc - 1e-9) {
+        best = t;
+        bc = c;
+      }
+    }
+    tile[b] = best < 0 ? T64 : best;
+  }
+  // Undersubscription repair.
+  int64_t items = 0;
+  for (int b = 0; b < B; ++b) items += num_tiles(tile[b], sh[b]);
+  const int smaller[NTILES] = {-1, T16, T32, T64, T64, T128x64};
+  for (int iter = 0; iter < 4 * B && items < slots; ++iter) {
+    int pick = -1;
+    double longest = 0;
+    for (int b = 0; b < B; ++b) {
+      int s = smaller[tile[b]];
+      while (s >= 0 && !tile_allowed(prec, s, o.tile_mask)) s = smaller[s];
+      if (s < 0) continue;
+      const double c = item_cost(tile[b], sh[b]);
+      if (c > longest) {
+        longest = c;
+        pick = b;
+      }
+    }
+    if (pick < 0) break;
+    int s = smaller[tile[pick]];
+    while (s >= 0 && !tile_allowed(prec, s, o.tile_mask)) s = smaller[s];
+    items += num_tiles(s, sh[pick]) - num_tiles(tile[pick], sh[pick]);
+    tile[pick] = s;
+  }
+}
+
+// Reference tile index (gemm.cuh:121-134, M-tile x N-tile) -> this library's
+// smallest tile covering it.
+const int kRefToTile[5] = {T16, T32, T32, T64, T64};
+
+// Wang (ROCm/tiling/gemm_wang.cuh:104-182): one global square tile from
+// {16, 32, 64} -- the first whose TLP = sum (M/t)(N/t)*128 falls below
+// tlp_thres -- then per matrix the largest dividing square tile not above it,
+// then demotion from the batch end while the workgroup count is below
+// tlp_thres (the reference's second threshold; one threshold is used here).
+void choose_wang(const std::vector<Shape>& sh, int thres, std::vector<int>& tile) {
+  const int B = (int)sh.size();
+  const int ts[3] = {16, 32, 64};
+  int t;
+  for (t = 0; t < 3; ++t) {
+    int64_t tlp = 0;
+    for (int j = 0; j < B; ++j) tlp += (int64_t)(sh[j].M / ts[t]) * (sh[j].N / ts[t]) * 128;
+    if (tlp < thres) break;
+  }
+  if (t >= 3) t = 2;
+  std::vector<int> st(B, 0);
+  for (int j = 0; j < B; ++j) {
+    auto fits = [&](int k) {
+      return ts[k] <= sh[j].M && ts[k] <= sh[j].N && sh[j].M % ts[k] == 0 && sh[j].N % ts[k] == 0;
+    };
+    if (fits(t)) {
+      st[j] = t;
+    } else {
+      for (int k = 0; k < t; ++k)
+        if (fits(k)) st[j] = k;
+    }
+  }
+  int64_t wg = 0;
+  for (int j = 0; j < B; ++j) wg += (int64_t)cdiv(sh[j].M, ts[st[j]]) * cdiv(sh[j].N, ts[st[j]]);
+  for (int j = B - 1; j >= 0 && wg < thres; --j) {
+    if (st[j] == 0) continue;
+    wg += (int64_t)cdiv(sh[j].M, ts[st[j]]) * cdiv(sh[j].N, ts[st[j]]) * 3;
+    st[j] -= 1;
+  }
+  const int map[3] = {T16, T32, T64};
+  for (int j = 0; j < B; ++j) tile[j] = map[st[j]];
+}
+
+// Li (CUDA/batching/gemm.cu:99-127): global tile over the six-tile set, the
+// first whose TLP = sum (M/tm)(N/tn)*256 falls below tlp_thres; per matrix that
+// tile if it fits (tm <= M, tn <= N), else the last smaller tile with
+// tm == M and tn <= N, else 16x16.
+void choose_li(const std::vector<Shape>& sh, int thres, std::vector<int>& tile) {
+  const int B = (int)sh.size();
+  int t;
+  for (t = 0; t < NTILES; ++t) {
+    int64_t tlp = 0;
+    for (int j = 0; j < B; ++j)
+      tlp += (int64_t)(sh[j].M / kTileBM[t]) * (sh[j].N / kTileBN[t]) * 256;
+    if (tlp < thres) break;
+  }
+  if (t == NTILES) t = NTILES - 1;
+  for (int j = 0; j < B; ++j) {
+    int s = T16;
+    if (kTileBM[t] <= sh[j].M && kTileBN[t] <= sh[j].N) {
+      s = t;
+    } else {
+      for (int k = 0; k < t; ++k)
+        if (kTileBM[k] == sh[j].M && kTileBN[k] <= sh[j].N) s = k;
+    }
+    tile[j] = s;
+  }
+}
+
+}  // namespace
+
+int ref_plan(int batch, const int* M, const int* N, const int* K, int thres, int* t_strategy,
+             int* kernel_thread, int* grid, int64_t* wavefronts) {
+  (void)K;
+  static const int tsz[5][2] = {{16, 16}, {16, 32}, {32, 32}, {32, 64}, {64, 64}};
+  auto threads = [](int t) { return t <= 2 ? 128 : 256; };
+  if (batch <= 0 || !M || !N || !t_strategy || !kernel_thread || !grid) return VBG_ERR_INVALID_ARG;
+  for (int i = 0; i < batch; ++i) {
+    int c = -1;
+    for (int j = 4; j >= 0 && c < 0; --j)
+      if (tsz[j][0] <= M[i] && tsz[j][1] <= N[i] && M[i] % tsz[j][0] == 0 && N[i] % tsz[j][1] == 0)
+        c = j;
+    if (c < 0) return VBG_ERR_UNDEFINED;
+    t_strategy[i] = c;
+  }
+  int total = 0;
+  for (int i = 0; i < batch; ++i) {
+    const int t = t_strategy[i];
+    kernel_thread[i] = threads(t);
+    total += M[i] * N[i] / (tsz[t][0] * tsz[t][1]) * (kernel_thread[i] / 64);
+  }
+  if (wavefronts) wavefronts[0] = total;
+  for (int j = 4; j > 0 && total <= thres; --j) {
+    for (int i = 0; i < batch && total <= thres; ++i) {
+      if (t_strategy[i] != j) continue;
+      const int t = t_strategy[i];
+      // gemm.cuh:180 reads the thread count at index t (a tile number).
+      if (t >= batch) return VBG_ERR_UNDEFINED;
+      const int old_v = M[i] * N[i] / (tsz[t][0] * tsz[t][1]) * (kernel_thread[t] / 64);
+      const int nt = t > 0 ? t - 1 : 0;
+      t_strategy[i] = nt;
+      kernel_thread[i] = threads(nt);
+      const int new_v = M[i] * N[i] / (tsz[nt][0] * tsz[nt][1]) * (kernel_thread[i] / 64);
+      total += new_v - old_v;
+    }
+  }
+  if (wavefronts) wavefronts[1] = total;
+  int gx = 0, gy = 0;
+  for (int i = 0; i < batch; ++i) {
+    gx = std::max(gx, M[i] / tsz[t_strategy[i]][0]);
+    gy = std::max(gy, N[i] / tsz[t_strategy[i]][1]);
+  }
+  grid[0] = gx;
+  grid[1] = gy;
+  grid[2] = batch;
+  return VBG_OK;
+}
+
+int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K, const int* lda,
+               const int* ldb, const int* ldc, bool ta, bool tb, int prec, const vbg_opts& o,
+               int num_cus, int (*occupancy)(int, int, bool, bool), std::string& err) {
+  hp = HostPlan();
+  hp.batch = batch;
+  hp.ta = ta;
+  hp.tb = tb;
+  hp.prec = prec;
+  if (batch < 0) {
+    err = "batch < 0";
+    return VBG_ERR_INVALID_ARG;
+  }
+  if (batch > 0 && (!M || !N || !K || !lda || !ldb || !ldc)) {
+    err = "null dimension array";
+    return VBG_ERR_INVALID_ARG;
+  }
+  if (prec < VBG_PREC_FP32 || prec > VBG_PREC_FP32_SEQ) {
+    err = "unknown precision";
+    return VBG_ERR_INVALID_ARG;
+  }
+  hp.desc.resize(batch);
+  std::vector<Shape> sh(batch);
+  for (int b = 0; b < batch; ++b) {
+    if (M[b] < 0 || N[b] < 0 || K[b] < 0) {
+      err = "negative dimension at matrix " + std::to_string(b);
+      return VBG_ERR_INVALID_ARG;
+    }
+    const int rows_a = ta ? K[b] : M[b];
+    const int rows_b = tb ? N[b] : K[b];
+    if (lda[b] < std::max(1, rows_a) || ldb[b] < std::max(1, rows_b) || ldc[b] < std::max(1, M[b])) {
+      err = "leading dimension too small at matrix " + std::to_string(b);
+      return VBG_ERR_INVALID_ARG;
+    }
+    sh[b] = {M[b], N[b], K[b]};
+    hp.desc[b] = {M[b], N[b], K[b], lda[b], ldb[b], ldc[b], 0, 1};
+  }
+
+  const int occ_default = 2;
+  auto occ = [&](int fam, int tile) {
+    int v = occupancy ? occupancy(fam, tile, ta, tb) : occ_default;
+    return v > 0 ? v : 1;
+  };
+  const int cus = num_cus > 0 ? num_cus : 256;
+
+  // 1. tile per matrix
+  std::vector<int> tile(batch, T64);
+  if (prec == VBG_PREC_FP32_SEQ) {
+    std::fill(tile.begin(), tile.end(), T32);
+  } else {
+    switch (o.policy) {
+      case VBG_POLICY_AUTO:
+        choose_auto(sh, prec, o, cus * occ(FAM_FP32, T128), tile);
+        break;
+      case VBG_POLICY_FIXED:
+        if (o.fixed_tile < 0 || o.fixed_tile >= NTILES) {
+          err = "fixed_tile out of range";
+          return VBG_ERR_INVALID_ARG;
+        }
+        std::fill(tile.begin(), tile.end(), o.fixed_tile);
+        break;
+      case VBG_POLICY_REFERENCE: {
+        std::vector<int> t(batch), kt(batch);
+        int grid[3];
+        int rc = batch > 0 ? ref_plan(batch, M, N, K, o.wavefront_thres, t.data(), kt.data(), grid,
+                                      nullptr)
+                           : VBG_OK;
+        if (rc == VBG_OK) {
+          for (int b = 0; b < batch; ++b) tile[b] = kRefToTile[t[b]];
+        } else {
+          // the reference is undefined on this batch: AUTO instead
+          choose_auto(sh, prec, o, cus * occ(FAM_FP32, T128), tile);
+        }
+        break;
+      }
+      case VBG_POLICY_WANG:
+        choose_wang(sh, o.tlp_thres, tile);
+        break;
+      case VBG_POLICY_LI:
+        choose_li(sh, o.tlp_thres, tile);
+        break;
+      default:
+        err = "unknown policy";
+        return VBG_ERR_INVALID_ARG;
+    }
+  }
+  hp.tile_of.assign(batch, -1);
+
+  // 2. group matrices by kernel family/tile; 3. LPT over XCD queues.
+  hp.nq = o.xcd_queues ? VBG_NUM_XCD : 1;
+  const int fam = prec == VBG_PREC_FP32_SEQ ? FAM_SEQ : (prec == VBG_PREC_FP32 ? FAM_FP32 : FAM_SPLIT);
+  for (int t = 0; t < NTILES; ++t) {
+    std::vector<int> mats;
+    for (int b = 0; b < batch; ++b)
+      if (tile[b] == t && M[b] > 0 && N[b] > 0) mats.push_back(b);
+    if (mats.empty()) continue;
+    for (int b : mats) {
+      hp.tile_of[b] = t;
+      hp.desc[b].tiles_n = cdiv(N[b], kTileBN[t]);
+    }
+    Launch L;
+    L.tile = t;
+    L.family = fam;
+    // longest matrices first, ties by index for determinism
+    std::stable_sort(mats.begin(), mats.end(), [&](int x, int y) {
+      return matrix_cost(t, sh[x]) > matrix_cost(t, sh[y]);
+    });
+    std::vector<std::vector<int>> qm(hp.nq);
+    std::vector<double> load(hp.nq, 0.0);
+    for (int b : mats) {
+      const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      qm[q].push_back(b);
+      load[q] += matrix_cost(t, sh[b]);
+      L.cost += matrix_cost(t, sh[b]);
+    }
+    L.q_begin.push_back(0);
+    for (int q = 0; q < hp.nq; ++q) {
+      // per-item cost decreasing; a matrix's tiles stay together
+      std::stable_sort(qm[q].begin(), qm[q].end(), [&](int x, int y) {
+        return item_cost(t, sh[x]) > item_cost(t, sh[y]);
+      });
+      for (int b : qm[q]) {
+        const int tm_n = cdiv(M[b], kTileBM[t]), tn_n = cdiv(N[b], kTileBN[t]);
+        for (int i = 0; i < tm_n; ++i)
+          for (int j = 0; j < tn_n; ++j) L.items.push_back({b, i, j, 0});
+      }
+      L.q_begin.push_back((int)L.items.size());
+    }
+    const int per_cu = occ(fam, t);
+    int grid = o.grid_wgs > 0 ? o.grid_wgs : cus * per_cu;
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)L.items.size()));
+    L.grid = grid;
+    hp.launches.push_back(std::move(L));
+  }
+  std::stable_sort(hp.launches.begin(), hp.launches.end(),
+                   [](const Launch& x, const Launch& y) { return x.cost > y.cost; });
+  hp.ws_off.assign(batch, -1);
+  return VBG_OK;
+}
+
+}  // namespace vbg

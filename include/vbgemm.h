@@ -1,0 +1,186 @@
+/*
+ * vbgemm.h -- C ABI of the MI355X (gfx950) load-balanced variable-size batched
+ * GEMM library (libvbgemm.so).
+ *
+ * Plain C types only: device pointers are `const float* const*` arrays that
+ * live in device memory (as in the reference's dev_A/dev_B/dev_C), dims and
+ * leading dimensions are host `int` arrays, streams are `void*` holding a
+ * hipStream_t (NULL = the default stream).  Every entry point returns a
+ * vbg_status (0 on success, negative on error) and never exits the process;
+ * vbg_status_string() names a code and vbg_last_error() gives the detail of the
+ * last error raised on the calling thread.
+ *
+ * GEMM semantics are hipBLAS column-major (ROCm/default/gemm.cuh:82:
+ * hipblasSgemm(handle, OP_N, OP_N, M, N, K, &alpha, A, lda=M, B, ldb=K,
+ * &beta, C, ldc=M)):  C_b = alpha * op(A_b) * op(B_b) + beta * C_b with op(A_b)
+ * M_b x K_b and op(B_b) K_b x N_b.  beta == 0 does not read C.
+ *
+ * Which reference interface each entry point replaces is stated at each
+ * declaration (paths relative to the reference root).
+ */
+#ifndef VBGEMM_H
+#define VBGEMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VBG_ABI_VERSION 1
+
+typedef enum vbg_status {
+  VBG_OK = 0,
+  VBG_ERR_INVALID_ARG = -1,   /* bad dimension, leading dimension, pointer or option */
+  VBG_ERR_HIP = -2,           /* a HIP runtime call failed */
+  VBG_ERR_NOT_SUPPORTED = -3, /* valid request this build does not implement */
+  VBG_ERR_ALLOC = -4,         /* host or device allocation failed */
+  VBG_ERR_PLAN_MISMATCH = -5, /* plan used with the wrong entry point or device */
+  VBG_ERR_UNDEFINED = -6      /* the reference planner's behaviour is undefined here */
+} vbg_status;
+
+/* Arithmetic of a plan.  All paths take FP32 operands and write FP32 C. */
+typedef enum vbg_prec {
+  VBG_PREC_FP32 = 0,        /* FP32 MFMA (v_mfma_f32_32x32x2_f32), FP32 accumulate */
+  VBG_PREC_BF16X3 = 1,      /* x = hi + lo in bf16; hi*hi + hi*lo + lo*hi on bf16 MFMA */
+  VBG_PREC_FP16_SCALED = 2, /* per-matrix power-of-two scale into FP16, FP16 MFMA */
+  VBG_PREC_FP8_E4M3 = 3,    /* per-matrix amax scale (448/amax) into OCP e4m3, FP8 MFMA */
+  VBG_PREC_FP32_SEQ = 4     /* one sequential fma chain per element over ascending k
+                               (VALU); with alpha = beta = 1 it is seeded from C and is
+                               bit-identical to the reference's correct tiles */
+} vbg_prec;
+
+/* How the planner picks each matrix's tile and orders the work map. */
+typedef enum vbg_policy {
+  VBG_POLICY_AUTO = 0,      /* MI355X cost model over the tile family */
+  VBG_POLICY_REFERENCE = 1, /* reference tile choice + wavefront cascade
+                               (ROCm/framework/gemm.cuh:141-201) mapped onto this
+                               library's tiles; matrices the reference cannot tile
+                               fall back to AUTO */
+  VBG_POLICY_FIXED = 2,     /* every matrix uses opts.fixed_tile */
+  VBG_POLICY_WANG = 3,      /* Wang's global TLP-chosen square tile
+                               (ROCm/tiling/gemm_wang.cuh:115-182) */
+  VBG_POLICY_LI = 4         /* Li's global TLP tile over the 6-tile set
+                               (CUDA/batching/gemm.cu:99-127) */
+} vbg_policy;
+
+/* Tile shapes of the kernel family (index = tile id; BM x BN output tile). */
+typedef enum vbg_tile {
+  VBG_TILE_16x16 = 0,
+  VBG_TILE_32x32 = 1,
+  VBG_TILE_64x64 = 2,
+  VBG_TILE_128x64 = 3,
+  VBG_TILE_64x128 = 4,
+  VBG_TILE_128x128 = 5,
+  VBG_TILE_COUNT = 6
+} vbg_tile;
+
+typedef struct vbg_opts {
+  int policy;           /* vbg_policy */
+  int wavefront_thres;  /* REFERENCE: cascade threshold (reference default 6000) */
+  int tlp_thres;        /* WANG / LI: TLP threshold */
+  int fixed_tile;       /* FIXED: vbg_tile id */
+  unsigned tile_mask;   /* AUTO: bit t enables tile t (0 = all) */
+  int big_tile_min_mn;  /* AUTO: min(M,N) at which 128x128 becomes eligible (0 = default) */
+  int split_k_min;      /* AUTO: split K for matrices with K >= this and few tiles (0 = default,
+                           <0 = never split) */
+  int grid_wgs;         /* persistent grid size per kernel (0 = CUs x occupancy) */
+  int xcd_queues;       /* 1 = XCD-aware per-XCD work queues with stealing, 0 = one queue */
+  int host_only;        /* 1 = plan on the host only (no device upload, cannot run);
+                           for inspection and tests on machines without a GPU */
+  int num_cus;          /* host_only: CU count to plan for (0 = 256) */
+} vbg_opts;
+
+typedef struct vbg_plan vbg_plan;
+
+/* Fills *opts with the defaults (AUTO policy, XCD queues on, thres 6000). */
+int vbg_opts_default(vbg_opts* opts);
+
+const char* vbg_status_string(int status);
+const char* vbg_last_error(void);
+int vbg_abi_version(void);
+
+/*
+ * Creates a plan for `batch` column-major GEMMs of the given shapes.
+ * Replaces the reference's host planner and uploads
+ * (ROCm/framework/gemm.cuh:86-105 dims upload, :141-229 planner, :208-232
+ * strategy upload): M, N, K, lda, ldb, ldc are host arrays; the plan copies
+ * what it needs to device memory it owns, so the launch allocates nothing.
+ * transA / transB: 'N' or 'T' for the whole batch.  Requirements: M,N,K >= 0;
+ * lda >= max(1, rows of stored A), likewise ldb, ldc >= max(1, M).
+ */
+int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, const int* K,
+                    const int* lda, const int* ldb, const int* ldc, char transA, char transB,
+                    int prec, const vbg_opts* opts);
+
+int vbg_plan_destroy(vbg_plan* plan);
+
+/*
+ * Runs the plan: C_b = alpha*op(A_b)*op(B_b) + beta*C_b for every b.
+ * Replaces the framework launch hipLaunchKernelGGL(gemm<256>, ...,
+ * dev_M, dev_N, dev_K, dev_A, dev_B, dev_C, dev_T) (ROCm/framework/gemm.cuh:
+ * 246-247, kernel ROCm/framework/kernel.h:9-42) and the per-matrix loop
+ * hipblasSgemm(OP_N, OP_N, ...) (ROCm/default/gemm.cuh:79-83).
+ * dA, dB, dC: device arrays of `batch` device pointers.  Asynchronous on
+ * `stream`, allocates nothing, issues no memset: safe under hipGraph capture.
+ * Launches that share a plan must not overlap in time (the plan's work-queue
+ * counters are shared); same-stream launches never overlap.
+ */
+int vbg_sgemm_vbatched(vbg_plan* plan, float alpha, const float* const* dA,
+                       const float* const* dB, float beta, float* const* dC, void* stream);
+
+/* One-shot form: plan, run and destroy (synchronises the stream before the
+ * plan's device buffers are freed).  Same semantics as above. */
+int vbg_sgemm_vbatched_oneshot(int batch, const int* M, const int* N, const int* K,
+                               const int* lda, const int* ldb, const int* ldc, char transA,
+                               char transB, int prec, float alpha, const float* const* dA,
+                               const float* const* dB, float beta, float* const* dC,
+                               void* stream);
+
+/* Plan inspection (works on host_only plans).  what:
+ *   0 number of work items             1 number of kernel launches (families used)
+ *   2 tile id per matrix (out[batch])  3 persistent grid size of launch 0
+ *   4 work items as (mat, tile_m, tile_n, k_begin, k_end) x n (out[5*n])
+ *   5 split-K parts per matrix (out[batch])
+ * Returns the number of ints written (or the scalar for 0, 1, 3). */
+int vbg_plan_query(const vbg_plan* plan, int what, int* out, int out_len);
+
+/*
+ * Reference planner, restated (ROCm/framework/gemm.cuh:121-229): initial tile
+ * choice over {16x16,16x32,32x32,32x64,64x64}, wavefront accounting, the
+ * demotion cascade while total_wavefront <= wavefront_thres (including its
+ * kernel_thread[t_strategy[i]] read, gemm.cuh:180), and the bounding-box grid.
+ * Outputs t_strategy[batch], kernel_thread[batch], grid[3], wavefronts[2]
+ * (initial, final).  Returns VBG_ERR_UNDEFINED where the reference is
+ * undefined (no dividing tile, or the :180 read past the array).
+ */
+int vbg_ref_plan(int batch, const int* M, const int* N, const int* K, int wavefront_thres,
+                 int* t_strategy, int* kernel_thread, int* grid, int64_t* wavefronts);
+
+/* ------------------------------------------------------------------ grouped */
+/*
+ * Grouped GEMM for the differentiation path (no reference counterpart: the
+ * reference is forward FP32 only; this reuses the variable-size work map).
+ * Row-major BF16 operands, FP32 accumulate:
+ *   forward   C[rows_g, N]  = A[rows_g, K] * B_g[K, N]
+ *   backward  dA[rows_g, K] = dC[rows_g, N] * B_g^T
+ *             dB_g[K, N]    = A[rows_g, K]^T * dC[rows_g, N]
+ * A and C/dC/dA are stacked over groups (sum of M_g rows, group g starts at
+ * row offset sum_{h<g} M_h); B / dB are [G, K, N].  dtype codes: 0 = fp32,
+ * 1 = bf16 for the C / dC / dA / dB buffers.  Groups with M_g = 0 write zero dB.
+ */
+typedef struct vbg_grouped_plan vbg_grouped_plan;
+int vbg_grouped_plan_create(vbg_grouped_plan** plan, int G, const int* M_host, int K, int N,
+                            const vbg_opts* opts);
+int vbg_grouped_plan_destroy(vbg_grouped_plan* plan);
+int vbg_grouped_gemm_fwd(vbg_grouped_plan* plan, const void* A_bf16, const void* B_bf16,
+                         void* C, int c_dtype, void* stream);
+int vbg_grouped_gemm_bwd(vbg_grouped_plan* plan, const void* A_bf16, const void* B_bf16,
+                         const void* dC, int dc_dtype, void* dA, void* dB, int grad_dtype,
+                         void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VBGEMM_H */

@@ -1,0 +1,131 @@
+"""C-ABI library on the CPU: exports, host-only planning, argument checks,
+and the library's reference planner against the oracle's (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+DATA = os.path.join(HERE, "golden", "data_MN_K_512_128")
+
+
+def header_symbols():
+    src = open(os.path.join(ROOT, "include", "vbgemm.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(vbg_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(bg):
+    syms = header_symbols()
+    assert len(syms) >= 14
+    L = ctypes.CDLL(bg.LIB_PATH)
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(syms) == sorted(bg._lib.EXPORTS)
+    assert bg.lib().vbg_abi_version() == 1
+
+
+def test_status_strings(bg):
+    L = bg.lib()
+    for code, name in [(0, b"VBG_OK"), (-1, b"VBG_ERR_INVALID_ARG"), (-6, b"VBG_ERR_UNDEFINED")]:
+        assert L.vbg_status_string(code) == name
+
+
+def _cover(plan, M, N):
+    """Every (matrix, tile) covered exactly once, no item outside its matrix."""
+    items = plan.items()
+    tiles = plan.tiles()
+    shapes = __import__("batchgemm_amd").TILE_SHAPES
+    seen = set()
+    for mat, tm, tn, kb, ke in items:
+        bm, bn = shapes[tiles[mat]]
+        assert 0 <= tm * bm < M[mat] and 0 <= tn * bn < N[mat]
+        key = (mat, tm, tn, kb)
+        assert key not in seen
+        seen.add(key)
+    expect = sum(-(-int(m) // shapes[t][0]) * -(-int(n) // shapes[t][1])
+                 for m, n, t in zip(M, N, tiles) if m > 0 and n > 0)
+    assert len({(a, b, c) for a, b, c, _ in seen}) == expect
+
+
+@pytest.mark.parametrize("policy", ["auto", "reference", "wang", "li", "fixed"])
+def test_host_plan_covers_every_tile_once(bg, policy):
+    from batchgemm_amd import workloads as w
+    for seed, (M, N, K) in enumerate([w.shapes_uniform(64, 16, 256, 0),
+                                      w.shapes_unbalanced(512, seed=1),
+                                      w.shapes_loguniform(300, seed=2),
+                                      w.read_dataset(DATA, 256)]):
+        p = bg.Plan(M, N, K, host_only=True, policy=policy, tlp_thres=6000)
+        _cover(p, M, N)
+        assert p.n_items > 0 and 1 <= p.n_launches <= 6
+
+
+def test_host_plan_zero_sized_matrices(bg):
+    p = bg.Plan([0, 5, 7], [3, 0, 9], [4, 4, 0], host_only=True)
+    t = p.tiles()
+    assert t[0] == -1 and t[1] == -1 and t[2] >= 0     # K = 0 still writes beta*C
+    p = bg.Plan([], [], [], host_only=True)
+    assert p.n_items == 0
+
+
+def test_host_plan_xcd_queues_balance(bg):
+    from batchgemm_amd import workloads as w
+    M, N, K = w.shapes_unbalanced(2048, seed=0)
+    p = bg.Plan(M, N, K, host_only=True)
+    assert p.grid >= 256
+    p1 = bg.Plan(M, N, K, host_only=True, xcd_queues=0)
+    assert p1.n_items == p.n_items
+
+
+def test_invalid_arguments_return_errors(bg):
+    from batchgemm_amd._lib import VbgError
+    with pytest.raises(VbgError) as e:
+        bg.Plan([4], [4], [-1], host_only=True)
+    assert e.value.code == -1
+    with pytest.raises(VbgError):
+        bg.Plan([4], [4], [4], lda=[3], host_only=True)       # lda < M
+    with pytest.raises(VbgError):
+        bg.Plan([4], [4], [4], ldc=[2], host_only=True)
+    with pytest.raises(VbgError):
+        bg.Plan([4], [4], [4], transA="X", host_only=True)
+    with pytest.raises(VbgError):
+        bg.Plan([4], [4], [4], prec=9, host_only=True)
+    with pytest.raises(VbgError):
+        bg.Plan([4], [4], [4], host_only=True, policy="fixed", fixed_tile=7)
+    p = bg.Plan([4], [4], [4], host_only=True)
+    L = bg.lib()
+    rc = L.vbg_sgemm_vbatched(p._p, 1.0, None, None, 0.0, None, None)
+    assert rc == -5  # host-only plan cannot run
+
+
+def test_library_ref_plan_matches_oracle(bg, orc):
+    M, N, K = orc.read_dataset(DATA)
+    rng = np.random.default_rng(5)
+    cases = [(M[:B], N[:B], K[:B], 6000) for B in (8, 16, 32, 64, 128, 256)]
+    for _ in range(100):
+        B = int(rng.integers(1, 200)); s = int(rng.integers(0, 4096 - B))
+        cases.append((M[s:s + B], N[s:s + B], K[s:s + B], int(rng.integers(0, 9000))))
+    cases.append(([64] * 6, [64] * 6, [32] * 6, 100))
+    cases.append(([20], [20], [16], 6000))
+    for m, n, k, th in cases:
+        a = bg.ref_plan(m, n, k, th)
+        b = orc.ref_plan(m, n, k, th)
+        assert (a is None) == (b is None)
+        if a is None:
+            continue
+        assert np.array_equal(a["t"], b["t"])
+        assert np.array_equal(a["kernel_thread"], b["kernel_thread"])
+        assert a["grid"] == b["grid"]
+        assert (a["wf_init"], a["wf_final"]) == (b["wf_init"], b["wf_final"])
+
+
+def test_reference_policy_follows_reference_planner(bg, orc):
+    M, N, K = orc.read_dataset(DATA)
+    for B in (8, 64, 256):
+        p = bg.Plan(M[:B], N[:B], K[:B], host_only=True, policy="reference", wavefront_thres=6000)
+        ref = orc.ref_plan(M[:B], N[:B], K[:B], 6000)
+        to_tile = [0, 1, 1, 2, 2]
+        assert np.array_equal(p.tiles(), np.array([to_tile[t] for t in ref["t"]]))
