@@ -28,11 +28,11 @@ int hip_fail(hipError_t e, const char* what) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int occupancy_cb(int family, int tile, bool ta, bool tb) {
+int occupancy_cb(int family, int prec, int tile, bool ta, bool tb) {
   switch (family) {
     case vbg::FAM_FP32: return vbg_occupancy_fp32(tile, ta, tb);
     case vbg::FAM_SEQ: return vbg_occupancy_seq(ta, tb);
-    case vbg::FAM_SPLIT: return vbg_occupancy_split(tile, ta, tb);
+    case vbg::FAM_SPLIT: return vbg_occupancy_split(prec, tile, ta, tb);
     default: return 1;
   }
 }
@@ -46,8 +46,9 @@ struct vbg_plan {
   bool host_only = false;
   void* dbuf = nullptr;  // one device allocation: descs, items, queues, counters, scales
   vbg_mat_desc* d_desc = nullptr;
-  float* d_scales = nullptr;  // [2 * batch] (FP16 / FP8 paths)
+  float* d_scales = nullptr;  // [4 * batch] (FP16 / FP8 paths)
   unsigned* d_amax = nullptr; // [2 * batch] float bits of per-matrix amax
+  vbg_chunk* d_chunks = nullptr;
   struct DevLaunch {
     vbg_item* items;
     int* q_begin;
@@ -140,9 +141,11 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     const size_t off_desc = off;
     off = align256(off + sizeof(vbg_mat_desc) * std::max(1, hp.batch));
     const size_t off_scales = off;
-    off = align256(off + sizeof(float) * 2 * std::max(1, hp.batch));
+    off = align256(off + sizeof(float) * 4 * std::max(1, hp.batch));
     const size_t off_amax = off;
     off = align256(off + sizeof(unsigned) * 2 * std::max(1, hp.batch));
+    const size_t off_chunks = off;
+    off = align256(off + sizeof(vbg_chunk) * hp.chunks.size());
     std::vector<size_t> oi, oq, oc;
     for (const auto& L : hp.launches) {
       oi.push_back(off);
@@ -154,6 +157,8 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     }
     std::vector<char> host(off, 0);
     if (hp.batch) std::memcpy(host.data() + off_desc, hp.desc.data(), sizeof(vbg_mat_desc) * hp.batch);
+    if (!hp.chunks.empty())
+      std::memcpy(host.data() + off_chunks, hp.chunks.data(), sizeof(vbg_chunk) * hp.chunks.size());
     for (size_t l = 0; l < hp.launches.size(); ++l) {
       const auto& L = hp.launches[l];
       std::memcpy(host.data() + oi[l], L.items.data(), sizeof(vbg_item) * L.items.size());
@@ -174,6 +179,7 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     p->d_desc = (vbg_mat_desc*)(base + off_desc);
     p->d_scales = (float*)(base + off_scales);
     p->d_amax = (unsigned*)(base + off_amax);
+    p->d_chunks = (vbg_chunk*)(base + off_chunks);
     for (size_t l = 0; l < hp.launches.size(); ++l)
       p->dl.push_back({(vbg_item*)(base + oi[l]), (int*)(base + oq[l]), (unsigned*)(base + oc[l])});
   }
@@ -206,8 +212,8 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
   hipStream_t s = (hipStream_t)stream;
 
   if (hp.prec == VBG_PREC_FP16_SCALED || hp.prec == VBG_PREC_FP8_E4M3) {
-    int r = vbg_launch_scales(hp.prec, hp.batch, hp.ta, hp.tb, p->d_desc, dA, dB, p->d_amax,
-                              p->d_scales, s);
+    int r = vbg_launch_scales(hp.prec, hp.batch, hp.ta, hp.tb, p->d_desc, p->d_chunks,
+                              (int)hp.chunks.size(), dA, dB, p->d_amax, p->d_scales, s);
     if (r) return fail(VBG_ERR_HIP, std::string("scale pre-pass launch: ") + hipGetErrorString(hipGetLastError()));
   }
   for (size_t l = 0; l < hp.launches.size(); ++l) {

@@ -18,6 +18,14 @@ struct vbg_item {
   int kslice;
 };
 
+// A slice of one stored operand for the per-matrix amax pre-pass (FP16 / FP8):
+// elements [begin, end) when the operand is packed (ld == rows), else columns.
+struct vbg_chunk {
+  int mat;
+  int op;  // 0 = A, 1 = B
+  int64_t begin, end;
+};
+
 // Arguments of every persistent GEMM kernel (passed by value).
 struct vbg_args {
   const vbg_mat_desc* desc;

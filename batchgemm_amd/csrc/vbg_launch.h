@@ -9,10 +9,11 @@ int vbg_occupancy_fp32(int tile, int ta, int tb);
 int vbg_launch_seq(int ta, int tb, const vbg_args* a, int grid, void* stream);
 int vbg_occupancy_seq(int ta, int tb);
 
-// Split-precision family (bf16x3, FP16 scaled, FP8 e4m3).
+// Split-precision family (bf16x3, FP16 scaled, FP8 e4m3); a->prec selects.
 int vbg_launch_split(int tile, int ta, int tb, const vbg_args* a, int grid, void* stream);
-int vbg_occupancy_split(int tile, int ta, int tb);
-// Per-matrix amax pre-pass and scale computation for FP16 / FP8.
+int vbg_occupancy_split(int prec, int tile, int ta, int tb);
+// Per-matrix amax pre-pass and scales (FP16 / FP8): zero amax, amax over the
+// chunks, then scales[4b..4b+2] = {sA, sB, 1/(sA*sB)}.
 int vbg_launch_scales(int prec, int batch, int ta, int tb, const vbg_mat_desc* desc,
-                      const float* const* A, const float* const* B, unsigned* amax, float* scales,
-                      void* stream);
+                      const vbg_chunk* chunks, int n_chunks, const float* const* A,
+                      const float* const* B, unsigned* amax, float* scales, void* stream);

@@ -52,9 +52,14 @@ int num_tiles(int tile, const Shape& s) {
 
 double matrix_cost(int tile, const Shape& s) { return num_tiles(tile, s) * item_cost(tile, s); }
 
+bool is_split(int prec) {
+  return prec == VBG_PREC_BF16X3 || prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3;
+}
+
+// The split-precision family is built on 32x32 MFMA blocks only (>= 64x64).
 bool tile_allowed(int prec, int tile, unsigned mask) {
   if (mask && !(mask & (1u << tile))) return false;
-  (void)prec;
+  if (is_split(prec) && tile < T64) return false;
   return true;
 }
 
@@ -223,7 +228,7 @@ int ref_plan(int batch, const int* M, const int* N, const int* K, int thres, int
 
 int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K, const int* lda,
                const int* ldb, const int* ldc, bool ta, bool tb, int prec, const vbg_opts& o,
-               int num_cus, int (*occupancy)(int, int, bool, bool), std::string& err) {
+               int num_cus, int (*occupancy)(int, int, int, bool, bool), std::string& err) {
   hp = HostPlan();
   hp.batch = batch;
   hp.ta = ta;
@@ -260,7 +265,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
 
   const int occ_default = 2;
   auto occ = [&](int fam, int tile) {
-    int v = occupancy ? occupancy(fam, tile, ta, tb) : occ_default;
+    int v = occupancy ? occupancy(fam, prec, tile, ta, tb) : occ_default;
     return v > 0 ? v : 1;
   };
   const int cus = num_cus > 0 ? num_cus : 256;
@@ -306,6 +311,8 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
         return VBG_ERR_INVALID_ARG;
     }
   }
+  if (is_split(prec))
+    for (int b = 0; b < batch; ++b) tile[b] = std::max(tile[b], (int)T64);
   hp.tile_of.assign(batch, -1);
 
   // 2. group matrices by kernel family/tile; 3. LPT over XCD queues.
@@ -357,6 +364,25 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   std::stable_sort(hp.launches.begin(), hp.launches.end(),
                    [](const Launch& x, const Launch& y) { return x.cost > y.cost; });
   hp.ws_off.assign(batch, -1);
+  if (prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3) {
+    const int64_t target = 1 << 16;  // elements per chunk
+    for (int b = 0; b < batch; ++b) {
+      if (M[b] == 0 || N[b] == 0 || K[b] == 0) continue;
+      for (int op = 0; op < 2; ++op) {
+        const int rows = op == 0 ? (ta ? K[b] : M[b]) : (tb ? N[b] : K[b]);
+        const int cols = op == 0 ? (ta ? M[b] : K[b]) : (tb ? K[b] : N[b]);
+        const int ld = op == 0 ? lda[b] : ldb[b];
+        if (ld == rows) {
+          const int64_t n = (int64_t)rows * cols;
+          for (int64_t s0 = 0; s0 < n; s0 += target) hp.chunks.push_back({b, op, s0, std::min(n, s0 + target)});
+        } else {
+          const int64_t per = std::max<int64_t>(1, target / rows);
+          for (int64_t c0 = 0; c0 < cols; c0 += per)
+            hp.chunks.push_back({b, op, c0, std::min<int64_t>(cols, c0 + per)});
+        }
+      }
+    }
+  }
   return VBG_OK;
 }
 

@@ -36,6 +36,7 @@ struct HostPlan {
   int nq = 1;
   int64_t ws_floats = 0;         // split-K workspace
   std::vector<int64_t> ws_off;   // per matrix (floats), -1 if not split
+  std::vector<vbg_chunk> chunks; // amax pre-pass slices (FP16 / FP8)
 };
 
 // Fills `hp` from the shapes and options.  `occupancy(tile)` gives the
@@ -43,7 +44,7 @@ struct HostPlan {
 // Returns 0 or a negative vbg_status, with `err` set.
 int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K, const int* lda,
                const int* ldb, const int* ldc, bool ta, bool tb, int prec, const vbg_opts& o,
-               int num_cus, int (*occupancy)(int family, int tile, bool ta, bool tb),
+               int num_cus, int (*occupancy)(int family, int prec, int tile, bool ta, bool tb),
                std::string& err);
 
 // Reference planner restated from ROCm/framework/gemm.cuh:121-229.

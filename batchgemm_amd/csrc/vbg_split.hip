@@ -1,10 +1,342 @@
-// vbg_split.hip -- split-precision family (bf16x3, FP16 scaled, FP8 e4m3).
-#include "vbg_tiles.h"
+// vbg_split.hip -- reduced-precision MFMA paths with FP32 operands in HBM and
+// FP32 accumulate, quantised on the HBM -> LDS path (no extra pass over the
+// operands except the per-matrix amax of the scaled modes):
+//   bf16x3 : x = hi + lo, hi = bf16(x), lo = bf16(x - hi) (x - hi is exact);
+//            C += hi_a*hi_b + hi_a*lo_b + lo_a*hi_b on v_mfma_f32_32x32x16_bf16
+//   fp16   : x * 2^e (per-matrix e: amax*2^e in [2^14, 2^15)) rounded to FP16,
+//            v_mfma_f32_32x32x16_f16, result scaled by 2^-(eA+eB)
+//   fp8    : x * (448/amax) rounded to OCP e4m3, v_mfma_f32_32x32x16_fp8_fp8,
+//            result scaled by 1/(sA*sB)
+// No counterpart in the reference (its kernels are FP32 only); the tiling,
+// work map and persistent driver are shared with the FP32 family.
 #include "vbg_launch.h"
+#include "vbg_tiles.h"
+#include "../../include/vbgemm.h"
 
-int vbg_launch_split(int, int, int, const vbg_args*, int, void*) { return -1; }
-int vbg_occupancy_split(int, int, int) { return 1; }
-int vbg_launch_scales(int, int, int, int, const vbg_mat_desc*, const float* const*,
-                      const float* const*, unsigned*, float*, void*) {
-  return -1;
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef _Float16 v4h __attribute__((ext_vector_type(4)));
+
+enum { MODE_BF16X3 = 0, MODE_FP16 = 1, MODE_FP8 = 2 };
+
+template <int MODE>
+struct Mode;
+template <>
+struct Mode<MODE_BF16X3> {
+  static constexpr int EB = 2, PLANES = 2;
+};
+template <>
+struct Mode<MODE_FP16> {
+  static constexpr int EB = 2, PLANES = 1;
+};
+template <>
+struct Mode<MODE_FP8> {
+  static constexpr int EB = 1, PLANES = 1;
+};
+
+// LDS layout per operand plane: [mn][k] with k contiguous; pitch = BK*EB + 16
+// bytes.  Lane half h of an MFMA wave owns k in [h*BK/2, (h+1)*BK/2); step s
+// of the 32x32x16 MFMA takes 8 of them, k = h*BK/2 + 8s + [0, 8).
+template <int BM, int BN, int MODE, bool TA, bool TB>
+struct SplitTile {
+  static constexpr int BK = 32;
+  static constexpr int EB = Mode<MODE>::EB, PLANES = Mode<MODE>::PLANES;
+  static constexpr int PITCH = BK * EB + 16;  // bytes per LDS row
+  static constexpr int PLANE_A = BM * PITCH, PLANE_B = BN * PITCH;
+  static constexpr int STAGE = PLANES * (PLANE_A + PLANE_B);  // bytes
+  static constexpr int LDS_FLOATS = 2 * STAGE / 4;
+  static constexpr int WTM = BM / 2, WTN = BN / 2;
+  static constexpr int TM = WTM / 32, TN = WTN / 32;
+  static_assert(TM >= 1 && TN >= 1, "split tile needs >= 64x64");
+  using LA = TileLoader<BM, BK, 4, TA>;  // P unused: stores are done here
+  using LB = TileLoader<BN, BK, 4, !TB>;
+
+  // Quantise 4 consecutive k of one row and write them to the plane(s).
+  template <class L>
+  __device__ __forceinline__ static void store(const L& ld, char* plane0, int plane_bytes, float s) {
+#pragma unroll
+    for (int v = 0; v < L::NV; ++v) {
+      int row, kq;
+      L::coords(v, row, kq);
+      const v4f x = ld.r[v];
+      char* dst = plane0 + row * PITCH + 4 * kq * EB;
+      if (MODE == MODE_BF16X3) {
+        const v4bf hi = {(__bf16)x[0], (__bf16)x[1], (__bf16)x[2], (__bf16)x[3]};
+        const v4bf lo = {(__bf16)(x[0] - (float)hi[0]), (__bf16)(x[1] - (float)hi[1]),
+                         (__bf16)(x[2] - (float)hi[2]), (__bf16)(x[3] - (float)hi[3])};
+        *(v4bf*)dst = hi;
+        *(v4bf*)(dst + plane_bytes) = lo;
+      } else if (MODE == MODE_FP16) {
+        const v4h q = {(_Float16)(x[0] * s), (_Float16)(x[1] * s), (_Float16)(x[2] * s),
+                       (_Float16)(x[3] * s)};
+        *(v4h*)dst = q;
+      } else {
+        int p = __builtin_amdgcn_cvt_pk_fp8_f32(x[0] * s, x[1] * s, 0, false);
+        p = __builtin_amdgcn_cvt_pk_fp8_f32(x[2] * s, x[3] * s, p, true);
+        *(int*)dst = p;
+      }
+    }
+  }
+
+  __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds_f) {
+    char* lds = (char*)lds_f;
+    const vbg_mat_desc d = a.desc[it.mat];
+    const float* __restrict__ A = a.A[it.mat];
+    const float* __restrict__ B = a.B[it.mat];
+    float* __restrict__ C = a.C[it.mat];
+    const int m0 = it.tm * BM, n0 = it.tn * BN;
+    int kb, ke;
+    vbg_k_range<BK>(d.K, d.ksplit, it.kslice, kb, ke);
+    const int nk = (ke - kb + BK - 1) / BK;
+    float sA = 1.0f, sB = 1.0f, post = 1.0f;
+    if (MODE != MODE_BF16X3) {
+      sA = a.scales[4 * it.mat];
+      sB = a.scales[4 * it.mat + 1];
+      post = a.scales[4 * it.mat + 2];
+    }
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
+    const int l32 = lane & 31, h = lane >> 5;
+
+    v16f acc[TM][TN];
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int s = 0; s < TN; ++s)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[r][s][e] = 0.0f;
+
+    LA la;
+    LB lb;
+    auto put = [&](char* stage) {
+      store(la, stage, PLANE_A, sA);
+      store(lb, stage + PLANES * PLANE_A, PLANE_B, sB);
+    };
+    if (nk > 0) {
+      const bool tail0 = kb + BK > ke;
+      la.load(A, d.lda, m0, kb, d.M, ke, tail0);
+      lb.load(B, d.ldb, n0, kb, d.N, ke, tail0);
+      put(lds);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) {
+        const int k0 = kb + (kt + 1) * BK;
+        const bool tail = k0 + BK > ke;
+        la.load(A, d.lda, m0, k0, d.M, ke, tail);
+        lb.load(B, d.ldb, n0, k0, d.N, ke, tail);
+      }
+      const char* As = lds + cur * STAGE;
+      const char* Bs = As + PLANES * PLANE_A;
+#pragma unroll
+      for (int s = 0; s < BK / 16; ++s) {
+        const int koff = (h * (BK / 2) + 8 * s) * EB;
+        if (MODE == MODE_FP8) {
+          long av[TM], bv[TN];
+#pragma unroll
+          for (int r = 0; r < TM; ++r) av[r] = *(const long*)(As + (wr * WTM + r * 32 + l32) * PITCH + koff);
+#pragma unroll
+          for (int c = 0; c < TN; ++c) bv[c] = *(const long*)(Bs + (wc * WTN + c * 32 + l32) * PITCH + koff);
+#pragma unroll
+          for (int r = 0; r < TM; ++r)
+#pragma unroll
+            for (int c = 0; c < TN; ++c)
+              acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(av[r], bv[c], acc[r][c], 0, 0, 0);
+        } else if (MODE == MODE_FP16) {
+          v8h av[TM], bv[TN];
+#pragma unroll
+          for (int r = 0; r < TM; ++r) av[r] = *(const v8h*)(As + (wr * WTM + r * 32 + l32) * PITCH + koff);
+#pragma unroll
+          for (int c = 0; c < TN; ++c) bv[c] = *(const v8h*)(Bs + (wc * WTN + c * 32 + l32) * PITCH + koff);
+#pragma unroll
+          for (int r = 0; r < TM; ++r)
+#pragma unroll
+            for (int c = 0; c < TN; ++c)
+              acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av[r], bv[c], acc[r][c], 0, 0, 0);
+        } else {
+          v8bf ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+          for (int r = 0; r < TM; ++r) {
+            const char* p = As + (wr * WTM + r * 32 + l32) * PITCH + koff;
+            ah[r] = *(const v8bf*)p;
+            al[r] = *(const v8bf*)(p + PLANE_A);
+          }
+#pragma unroll
+          for (int c = 0; c < TN; ++c) {
+            const char* p = Bs + (wc * WTN + c * 32 + l32) * PITCH + koff;
+            bh[c] = *(const v8bf*)p;
+            bl[c] = *(const v8bf*)(p + PLANE_B);
+          }
+#pragma unroll
+          for (int r = 0; r < TM; ++r)
+#pragma unroll
+            for (int c = 0; c < TN; ++c) {
+              acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
+              acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh[c], acc[r][c], 0, 0, 0);
+              acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
+            }
+        }
+      }
+      if (more) put(lds + (cur ^ 1) * STAGE);
+      __syncthreads();
+    }
+    const float alpha = a.alpha * post;
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int c = 0; c < TN; ++c)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
+          const int col = n0 + wc * WTN + c * 32 + l32;
+          v4f v = {acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
+          vbg_store4(C, d.ldc, row, col, d.M, d.N, v, alpha, a.beta);
+        }
+  }
+};
+
+template <class Tile>
+__global__ void __launch_bounds__(VBG_THREADS) vbg_split_kernel(vbg_args a) {
+  __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
+  vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
+}
+
+// ---------------------------------------------------------------- amax / scales
+// Chunk of one stored operand: elements [begin, end) of the packed matrix
+// (ld == rows), or columns [begin, end) otherwise.
+__device__ __forceinline__ float block_max(float m) {
+  __shared__ float red[VBG_THREADS / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < VBG_THREADS / 64; ++w) m = fmaxf(m, red[w]);
+  return m;
+}
+
+__global__ void __launch_bounds__(VBG_THREADS) vbg_amax_zero(unsigned* amax, int n) {
+  const int i = blockIdx.x * VBG_THREADS + threadIdx.x;
+  if (i < n) amax[i] = 0u;
+}
+
+__global__ void __launch_bounds__(VBG_THREADS)
+    vbg_amax_kernel(const vbg_chunk* ch, const vbg_mat_desc* desc, const float* const* A,
+                    const float* const* B, int ta, int tb, unsigned* amax) {
+  const vbg_chunk c = ch[blockIdx.x];
+  const vbg_mat_desc d = desc[c.mat];
+  const float* p = c.op == 0 ? A[c.mat] : B[c.mat];
+  const int rows = c.op == 0 ? (ta ? d.K : d.M) : (tb ? d.N : d.K);
+  const int ld = c.op == 0 ? d.lda : d.ldb;
+  float m = 0.0f;
+  if (ld == rows) {
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += VBG_THREADS) m = fmaxf(m, fabsf(p[i]));
+  } else {
+    for (int64_t col = c.begin; col < c.end; ++col)
+      for (int r = threadIdx.x; r < rows; r += VBG_THREADS) m = fmaxf(m, fabsf(p[r + col * ld]));
+  }
+  m = block_max(m);
+  if (threadIdx.x == 0) atomicMax(&amax[2 * c.mat + c.op], __float_as_uint(m));
+}
+
+// scales[4b..4b+2] = {sA, sB, 1/(sA*sB)}
+__global__ void __launch_bounds__(VBG_THREADS)
+    vbg_scales_kernel(int prec, int batch, const unsigned* amax, float* scales) {
+  const int b = blockIdx.x * VBG_THREADS + threadIdx.x;
+  if (b >= batch) return;
+  float s[2];
+#pragma unroll
+  for (int op = 0; op < 2; ++op) {
+    const float m = __uint_as_float(amax[2 * b + op]);
+    if (!(m > 0.0f) || isinf(m)) {
+      s[op] = 1.0f;
+    } else if (prec == VBG_PREC_FP16_SCALED) {
+      int x;
+      frexpf(m, &x);  // m = f * 2^x, f in [0.5, 1)
+      s[op] = ldexpf(1.0f, 15 - x);
+    } else {
+      s[op] = 448.0f / m;
+    }
+  }
+  scales[4 * b] = s[0];
+  scales[4 * b + 1] = s[1];
+  scales[4 * b + 2] = 1.0f / (s[0] * s[1]);
+  scales[4 * b + 3] = 0.0f;
+}
+
+}  // namespace
+
+template <class Tile>
+static int run_or_occ(const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  if (occ) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, vbg_split_kernel<Tile>, VBG_THREADS, 0) !=
+        hipSuccess)
+      nb = 1;
+    *occ = nb > 0 ? nb : 1;
+    return 0;
+  }
+  hipLaunchKernelGGL(vbg_split_kernel<Tile>, dim3(grid), dim3(VBG_THREADS), 0, s, *a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <int BM, int BN, int MODE>
+static int dispatch_tt(bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  if (!ta && !tb) return run_or_occ<SplitTile<BM, BN, MODE, false, false>>(a, grid, s, occ);
+  if (!ta && tb) return run_or_occ<SplitTile<BM, BN, MODE, false, true>>(a, grid, s, occ);
+  if (ta && !tb) return run_or_occ<SplitTile<BM, BN, MODE, true, false>>(a, grid, s, occ);
+  return run_or_occ<SplitTile<BM, BN, MODE, true, true>>(a, grid, s, occ);
+}
+
+template <int MODE>
+static int dispatch_tile(int tile, bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s,
+                         int* occ) {
+  switch (tile) {
+    case T64: return dispatch_tt<64, 64, MODE>(ta, tb, a, grid, s, occ);
+    case T128x64: return dispatch_tt<128, 64, MODE>(ta, tb, a, grid, s, occ);
+    case T64x128: return dispatch_tt<64, 128, MODE>(ta, tb, a, grid, s, occ);
+    case T128: return dispatch_tt<128, 128, MODE>(ta, tb, a, grid, s, occ);
+    default: return -1;
+  }
+}
+
+static int split_dispatch(int prec, int tile, bool ta, bool tb, const vbg_args* a, int grid,
+                          hipStream_t s, int* occ) {
+  switch (prec) {
+    case VBG_PREC_BF16X3: return dispatch_tile<MODE_BF16X3>(tile, ta, tb, a, grid, s, occ);
+    case VBG_PREC_FP16_SCALED: return dispatch_tile<MODE_FP16>(tile, ta, tb, a, grid, s, occ);
+    case VBG_PREC_FP8_E4M3: return dispatch_tile<MODE_FP8>(tile, ta, tb, a, grid, s, occ);
+    default: return -1;
+  }
+}
+
+int vbg_launch_split(int tile, int ta, int tb, const vbg_args* a, int grid, void* stream) {
+  return split_dispatch(a->prec, tile, ta != 0, tb != 0, a, grid, (hipStream_t)stream, nullptr);
+}
+
+int vbg_occupancy_split(int prec, int tile, int ta, int tb) {
+  int occ = 1;
+  split_dispatch(prec, tile, ta != 0, tb != 0, nullptr, 0, nullptr, &occ);
+  return occ;
+}
+
+int vbg_launch_scales(int prec, int batch, int ta, int tb, const vbg_mat_desc* desc,
+                      const vbg_chunk* chunks, int n_chunks, const float* const* A,
+                      const float* const* B, unsigned* amax, float* scales, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = (2 * batch + VBG_THREADS - 1) / VBG_THREADS;
+  hipLaunchKernelGGL(vbg_amax_zero, dim3(nb > 0 ? nb : 1), dim3(VBG_THREADS), 0, s, amax, 2 * batch);
+  if (n_chunks > 0)
+    hipLaunchKernelGGL(vbg_amax_kernel, dim3(n_chunks), dim3(VBG_THREADS), 0, s, chunks, desc, A, B,
+                       ta, tb, amax);
+  const int sb = (batch + VBG_THREADS - 1) / VBG_THREADS;
+  hipLaunchKernelGGL(vbg_scales_kernel, dim3(sb > 0 ? sb : 1), dim3(VBG_THREADS), 0, s, prec, batch,
+                     amax, scales);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -1,0 +1,106 @@
+"""GPU parity of the split-precision paths (needs an MI355X).
+
+Each path is checked twice:
+  * against a numpy emulation of its operand quantisation (bf16 hi/lo split,
+    FP16 with the per-matrix power-of-two scale, OCP e4m3 with the 448/amax
+    scale; the rounding helpers are pinned to torch's dtype conversions in
+    test_oracle.py) evaluated in FP64 -- this pins the kernel's quantisation,
+    scale and MFMA operand layout; the only difference left is FP32
+    accumulation order, hence the 2e-6 bar;
+  * against the FP64 oracle on the unquantised operands: <= 1e-5 for bf16x3
+    (the target of the path); FP16 and FP8 errors are printed and bounded
+    loosely (they are what the formats give).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def op_matrix(vb, which, b):
+    flat = vb.host_flat(which, b)
+    if which == "A":
+        rows, cols, ld, t = int(vb.a_rows[b]), int(vb.a_cols[b]), int(vb.lda[b]), vb.transA
+    else:
+        rows, cols, ld, t = int(vb.b_rows[b]), int(vb.b_cols[b]), int(vb.ldb[b]), vb.transB
+    X = flat.reshape(cols, ld)[:, :rows].T if cols else np.zeros((rows, 0), np.float32)
+    return X.T if t == "T" else X
+
+
+def emulate(orc, prec, A, B):
+    """Quantised operands and the post-scale, as the kernel builds them."""
+    if prec == "bf16x3":
+        ah, al = orc.bf16_split(A); bh, bl = orc.bf16_split(B)
+        d = lambda x: x.astype(np.float64)
+        return d(ah) @ d(bh) + d(ah) @ d(bl) + d(al) @ d(bh)
+    if prec == "fp16":
+        ea = orc.fp16_pow2_exponent(np.abs(A).max() if A.size else 0.0)
+        eb = orc.fp16_pow2_exponent(np.abs(B).max() if B.size else 0.0)
+        qa = orc.fp16_scaled(A, ea).astype(np.float64); qb = orc.fp16_scaled(B, eb).astype(np.float64)
+        return (qa @ qb) * np.float64(np.float32(1.0) / (np.float32(2.0 ** ea) * np.float32(2.0 ** eb)))
+    if prec == "fp8":
+        sa = orc.fp8_scale(np.abs(A).max() if A.size else 0.0)
+        sb = orc.fp8_scale(np.abs(B).max() if B.size else 0.0)
+        qa = orc.fp8_scaled(A, sa).astype(np.float64); qb = orc.fp8_scaled(B, sb).astype(np.float64)
+        return (qa @ qb) * np.float64(np.float32(1.0) / (sa * sb))
+    raise ValueError(prec)
+
+
+FP64_BOUND = {"bf16x3": 1e-5, "fp16": 2e-3, "fp8": 0.2}
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16", "fp8"])
+@pytest.mark.parametrize("trans", ["NN", "TN", "NT"])
+@pytest.mark.parametrize("outliers", [0.0, 0.01])
+def test_split_precision_paths(bg, orc, gpu, prec, trans, outliers):
+    import torch
+    from batchgemm_amd.workloads import VBatch
+    say(f"[{prec}] trans={trans} outliers={outliers}")
+    M = [64, 100, 17, 130, 1, 255, 96]
+    N = [64, 33, 140, 128, 7, 64, 1]
+    K = [32, 200, 9, 257, 5, 100, 64]
+    vb = VBatch(M, N, K, trans[0], trans[1]).fill("normal", seed=9, outlier_frac=outliers,
+                                                   c_dist="uniform")
+    C0 = vb.C.cpu().numpy()
+    alpha, beta = 1.5, 0.5
+    plan = bg.Plan(vb.M, vb.N, vb.K, vb.lda, vb.ldb, vb.ldc, trans[0], trans[1], prec=prec)
+    plan.run(vb.dA, vb.dB, vb.dC, alpha, beta)
+    torch.cuda.synchronize()
+    worst_emul = worst_fp64 = 0.0
+    for b in range(vb.batch):
+        A = op_matrix(vb, "A", b); B = op_matrix(vb, "B", b)
+        c0 = C0[int(vb.offC[b]): int(vb.offC[b]) + M[b] * N[b]].reshape(N[b], M[b]).T.astype(np.float64)
+        got = vb.host_flat("C", b).reshape(N[b], M[b]).T.astype(np.float64)
+        emul = alpha * emulate(orc, prec, A, B) + beta * c0
+        exact = alpha * (A.astype(np.float64) @ B.astype(np.float64)) + beta * c0
+        e1 = orc.rel_fro(got, emul); e2 = orc.rel_fro(got, exact)
+        worst_emul = max(worst_emul, e1); worst_fp64 = max(worst_fp64, e2)
+        assert e1 <= 2e-6, (b, M[b], N[b], K[b], e1)
+        assert e2 <= FP64_BOUND[prec], (b, e2)
+    say(f"  max rel err vs emulation {worst_emul:.3g}, vs FP64 {worst_fp64:.3g}")
+
+
+def test_split_repeated_launch_and_plan_reuse(bg, orc, gpu):
+    """The amax pre-pass re-zeroes its accumulators every launch: scaling the
+    inputs down between launches must change the scales."""
+    import torch
+    from batchgemm_amd.workloads import VBatch
+    say("[fp8] plan reuse")
+    M, N, K = [70, 64], [50, 64], [90, 64]
+    vb = VBatch(M, N, K).fill("normal", seed=4)
+    plan = bg.Plan(M, N, K, prec="fp8")
+    plan.run(vb.dA, vb.dB, vb.dC, 1.0, 0.0)
+    vb.A.mul_(1e-3)
+    plan.run(vb.dA, vb.dB, vb.dC, 1.0, 0.0)
+    torch.cuda.synchronize()
+    for b in range(2):
+        A = op_matrix(vb, "A", b); B = op_matrix(vb, "B", b)
+        got = vb.host_flat("C", b).reshape(N[b], M[b]).T
+        assert orc.rel_fro(got, emulate(orc, "fp8", A, B)) <= 2e-6
