@@ -33,6 +33,7 @@ int occupancy_cb(int family, int prec, int tile, bool ta, bool tb) {
     case vbg::FAM_FP32: return vbg_occupancy_fp32(tile, ta, tb);
     case vbg::FAM_SEQ: return vbg_occupancy_seq(ta, tb);
     case vbg::FAM_SPLIT: return vbg_occupancy_split(prec, tile, ta, tb);
+    case vbg::FAM_BF16: return vbg_occupancy_bf16(prec, tile, ta, tb);
     default: return 1;
   }
 }
@@ -240,6 +241,7 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
       case vbg::FAM_FP32: r = vbg_launch_fp32(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
       case vbg::FAM_SEQ: r = vbg_launch_seq(hp.ta, hp.tb, &a, L.grid, s); break;
       case vbg::FAM_SPLIT: r = vbg_launch_split(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
+      case vbg::FAM_BF16: r = vbg_launch_bf16(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
       default: r = -1;
     }
     if (r == -1) return fail(VBG_ERR_NOT_SUPPORTED, "tile/precision combination not built");

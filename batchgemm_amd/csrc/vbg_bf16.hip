@@ -1,0 +1,282 @@
+// vbg_bf16.hip -- BF16-operand tiles (v_mfma_f32_32x32x16_bf16, FP32
+// accumulate) for the grouped forward / backward passes.  Operands are BF16 in
+// HBM and are staged through LDS unconverted; C is written as FP32 or BF16.
+//
+// Loads are 16-byte vectors (8 BF16), so every leading dimension must be a
+// multiple of 8 and every operand pointer 16-byte aligned (the plan checks the
+// leading dimensions; the grouped entry points check the pointers).
+#include "vbg_launch.h"
+#include "vbg_tiles.h"
+#include "../../include/vbgemm.h"
+
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned u32;
+typedef u32 v4u __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+__device__ __forceinline__ u16 ld16(const u16* p) { return *p; }
+
+// BF16 operand loader.  LDS plane layout [mn][k], k contiguous, PITCH bytes.
+// KC (k contiguous in HBM): a thread loads 8 consecutive k of one row.
+// MC (mn contiguous in HBM): a thread loads 8 consecutive mn at k and at k+1,
+// then interleaves them into 8 (k, k+1) pairs, one 4-byte LDS write per row.
+// Rows past MN are loaded from clamped addresses (they only feed outputs that
+// are never stored); k past K is zero.
+template <int ROWS, int BK, int PITCH, bool KC>
+struct Bf16Loader {
+  static constexpr int UNITS = KC ? ROWS * BK / 8 : ROWS * BK / 16;
+  static constexpr int NV = UNITS / VBG_THREADS;
+  static_assert(NV >= 1 && UNITS % VBG_THREADS == 0, "tile too small for the block");
+  v4u r[KC ? NV : 2 * NV];
+
+  __device__ __forceinline__ static void coords(int v, int& row, int& k) {
+    const int idx = threadIdx.x + VBG_THREADS * v;
+    if (KC) {
+      row = idx / (BK / 8);
+      k = 8 * (idx % (BK / 8));
+    } else {
+      row = 8 * (idx % (ROWS / 8));
+      k = 2 * (idx / (ROWS / 8));
+    }
+  }
+
+  __device__ __forceinline__ void load(const u16* __restrict__ p, int ld, int mn0, int k0, int MN,
+                                       int K, bool ktail) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      int row, kk;
+      coords(v, row, kk);
+      const int k = k0 + kk;
+      if (KC) {
+        const int mn = min(mn0 + row, MN - 1);
+        const u16* q = p + (size_t)mn * ld;
+        if (!ktail || k + 7 < K) {
+          r[v] = *(const v4u*)(q + k);
+        } else {
+          u32 w[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int ka = k + 2 * e, kb = ka + 1;
+            const u32 lo = ka < K ? ld16(q + ka) : 0u;
+            const u32 hi = kb < K ? ld16(q + kb) : 0u;
+            w[e] = lo | (hi << 16);
+          }
+          r[v] = v4u{w[0], w[1], w[2], w[3]};
+        }
+      } else {
+        const int mn = mn0 + row;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int kj = k + j;
+          const bool okk = !ktail || kj < K;
+          const u16* q = p + (size_t)(okk ? kj : K - 1) * ld;
+          v4u x;
+          if (mn + 7 < MN) {
+            x = *(const v4u*)(q + mn);
+          } else {
+            u32 w[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const u32 lo = ld16(q + min(mn + 2 * e, MN - 1));
+              const u32 hi = ld16(q + min(mn + 2 * e + 1, MN - 1));
+              w[e] = lo | (hi << 16);
+            }
+            x = v4u{w[0], w[1], w[2], w[3]};
+          }
+          if (!okk) x = v4u{0u, 0u, 0u, 0u};
+          r[2 * v + j] = x;
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(char* plane) const {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      int row, kk;
+      coords(v, row, kk);
+      if (KC) {
+        *(v4u*)(plane + row * PITCH + 2 * kk) = r[v];
+      } else {
+        const v4u a = r[2 * v], b = r[2 * v + 1];  // k, k+1; 8 mn each
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const u32 even = (a[e] & 0xffffu) | (b[e] << 16);
+          const u32 odd = (a[e] >> 16) | (b[e] & 0xffff0000u);
+          *(u32*)(plane + (row + 2 * e) * PITCH + 2 * kk) = even;
+          *(u32*)(plane + (row + 2 * e + 1) * PITCH + 2 * kk) = odd;
+        }
+      }
+    }
+  }
+};
+
+__device__ __forceinline__ float bf2f(u16 x) { return __uint_as_float((u32)x << 16); }
+
+// 4-row column segment of a column-major BF16 C: round alpha*v + beta*C to BF16.
+__device__ __forceinline__ void store4_bf16(u16* C, int ldc, int row, int col, int M, int N, v4f v,
+                                            float alpha, float beta) {
+  if (col >= N || row >= M) return;
+  u16* c = C + (size_t)col * ldc + row;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (row + e < M) {
+      float o = alpha * v[e];
+      if (beta != 0.0f) o += beta * bf2f(c[e]);
+      const __bf16 h = (__bf16)o;
+      c[e] = *(const u16*)&h;
+    }
+  }
+}
+
+template <int BM, int BN, bool TA, bool TB, bool OUT_BF16>
+struct Bf16Tile {
+  static constexpr int BK = 64;
+  static constexpr int PITCH = BK * 2 + 16;
+  static constexpr int PLANE_A = BM * PITCH, PLANE_B = BN * PITCH;
+  static constexpr int STAGE = PLANE_A + PLANE_B;
+  static constexpr int LDS_FLOATS = 2 * STAGE / 4;
+  static constexpr int WTM = BM / 2, WTN = BN / 2;
+  static constexpr int TM = WTM / 32, TN = WTN / 32;
+  static_assert(TM >= 1 && TN >= 1, "BF16 tile needs >= 64x64");
+  using LA = Bf16Loader<BM, BK, PITCH, TA>;
+  using LB = Bf16Loader<BN, BK, PITCH, !TB>;
+
+  __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds_f) {
+    char* lds = (char*)lds_f;
+    const vbg_mat_desc d = a.desc[it.mat];
+    const u16* __restrict__ A = (const u16*)a.A[it.mat];
+    const u16* __restrict__ B = (const u16*)a.B[it.mat];
+    const int m0 = it.tm * BM, n0 = it.tn * BN;
+    const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
+    const int kb = kr.kb, ke = kr.ke;
+    const int nk = (ke - kb + BK - 1) / BK;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
+    const int l32 = lane & 31, h = lane >> 5;
+
+    v16f acc[TM][TN];
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int s = 0; s < TN; ++s)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[r][s][e] = 0.0f;
+
+    LA la;
+    LB lb;
+    if (nk > 0) {
+      const bool tail0 = kb + BK > ke;
+      la.load(A, d.lda, m0, kb, d.M, ke, tail0);
+      lb.load(B, d.ldb, n0, kb, d.N, ke, tail0);
+      la.store(lds);
+      lb.store(lds + PLANE_A);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) {
+        const int k0 = kb + (kt + 1) * BK;
+        const bool tail = k0 + BK > ke;
+        la.load(A, d.lda, m0, k0, d.M, ke, tail);
+        lb.load(B, d.ldb, n0, k0, d.N, ke, tail);
+      }
+      const char* As = lds + cur * STAGE;
+      const char* Bs = As + PLANE_A;
+#pragma unroll
+      for (int s = 0; s < BK / 16; ++s) {
+        const int koff = (h * (BK / 2) + 8 * s) * 2;
+        v8bf av[TM], bv[TN];
+#pragma unroll
+        for (int r = 0; r < TM; ++r) av[r] = *(const v8bf*)(As + (wr * WTM + r * 32 + l32) * PITCH + koff);
+#pragma unroll
+        for (int c = 0; c < TN; ++c) bv[c] = *(const v8bf*)(Bs + (wc * WTN + c * 32 + l32) * PITCH + koff);
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+#pragma unroll
+          for (int c = 0; c < TN; ++c)
+            acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[r], bv[c], acc[r][c], 0, 0, 0);
+      }
+      if (more) {
+        char* nxt = lds + (cur ^ 1) * STAGE;
+        la.store(nxt);
+        lb.store(nxt + PLANE_A);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int c = 0; c < TN; ++c)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
+          const int col = n0 + wc * WTN + c * 32 + l32;
+          v4f v = {acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
+          if (OUT_BF16)
+            store4_bf16((u16*)a.C[it.mat], d.ldc, row, col, d.M, d.N, v, a.alpha, a.beta);
+          else
+            vbg_store4(a.C[it.mat], d.ldc, row, col, d.M, d.N, v, a.alpha, a.beta);
+        }
+  }
+};
+
+template <class Tile>
+__global__ void __launch_bounds__(VBG_THREADS) vbg_bf16_kernel(vbg_args a) {
+  __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
+  vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
+}
+
+}  // namespace
+
+template <class Tile>
+static int run_or_occ(const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  if (occ) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, vbg_bf16_kernel<Tile>, VBG_THREADS, 0) !=
+        hipSuccess)
+      nb = 1;
+    *occ = nb > 0 ? nb : 1;
+    return 0;
+  }
+  hipLaunchKernelGGL(vbg_bf16_kernel<Tile>, dim3(grid), dim3(VBG_THREADS), 0, s, *a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <int BM, int BN, bool OB>
+static int dispatch_tt(bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  if (!ta && !tb) return run_or_occ<Bf16Tile<BM, BN, false, false, OB>>(a, grid, s, occ);
+  if (!ta && tb) return run_or_occ<Bf16Tile<BM, BN, false, true, OB>>(a, grid, s, occ);
+  if (ta && !tb) return run_or_occ<Bf16Tile<BM, BN, true, false, OB>>(a, grid, s, occ);
+  return run_or_occ<Bf16Tile<BM, BN, true, true, OB>>(a, grid, s, occ);
+}
+
+template <bool OB>
+static int dispatch_tile(int tile, bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s,
+                         int* occ) {
+  switch (tile) {
+    case T64: return dispatch_tt<64, 64, OB>(ta, tb, a, grid, s, occ);
+    case T128x64: return dispatch_tt<128, 64, OB>(ta, tb, a, grid, s, occ);
+    case T64x128: return dispatch_tt<64, 128, OB>(ta, tb, a, grid, s, occ);
+    case T128: return dispatch_tt<128, 128, OB>(ta, tb, a, grid, s, occ);
+    default: return -1;
+  }
+}
+
+int vbg_launch_bf16(int tile, int ta, int tb, const vbg_args* a, int grid, void* stream) {
+  const bool ob = a->prec == VBG_PREC_BF16_OUT_BF16;
+  return ob ? dispatch_tile<true>(tile, ta, tb, a, grid, (hipStream_t)stream, nullptr)
+            : dispatch_tile<false>(tile, ta, tb, a, grid, (hipStream_t)stream, nullptr);
+}
+
+int vbg_occupancy_bf16(int prec, int tile, int ta, int tb) {
+  int occ = 1;
+  if (prec == VBG_PREC_BF16_OUT_BF16)
+    dispatch_tile<true>(tile, ta, tb, nullptr, 0, nullptr, &occ);
+  else
+    dispatch_tile<false>(tile, ta, tb, nullptr, 0, nullptr, &occ);
+  return occ;
+}

@@ -42,8 +42,8 @@ struct Fp32MfmaTile {
     const float* __restrict__ B = a.B[it.mat];
     float* __restrict__ C = a.C[it.mat];
     const int m0 = it.tm * BM, n0 = it.tn * BN;
-    int kb, ke;
-    vbg_k_range<BK>(d.K, d.ksplit, it.kslice, kb, ke);
+    const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
+    const int kb = kr.kb, ke = kr.ke;
     const int nk = (ke - kb + BK - 1) / BK;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -149,8 +149,8 @@ struct Fp32SmallTile {
     const float* __restrict__ B = a.B[it.mat];
     float* __restrict__ C = a.C[it.mat];
     const int m0 = it.tm * BMN, n0 = it.tn * BMN;
-    int kb, ke;
-    vbg_k_range<BK>(d.K, d.ksplit, it.kslice, kb, ke);
+    const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
+    const int kb = kr.kb, ke = kr.ke;
     const int nk = (ke - kb + BK - 1) / BK;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, g = lane >> 4;

@@ -17,3 +17,7 @@ int vbg_occupancy_split(int prec, int tile, int ta, int tb);
 int vbg_launch_scales(int prec, int batch, int ta, int tb, const vbg_mat_desc* desc,
                       const vbg_chunk* chunks, int n_chunks, const float* const* A,
                       const float* const* B, unsigned* amax, float* scales, void* stream);
+
+// BF16-operand family (grouped passes); a->prec selects FP32 or BF16 C.
+int vbg_launch_bf16(int tile, int ta, int tb, const vbg_args* a, int grid, void* stream);
+int vbg_occupancy_bf16(int prec, int tile, int ta, int tb);

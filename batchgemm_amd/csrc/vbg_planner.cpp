@@ -53,10 +53,12 @@ int num_tiles(int tile, const Shape& s) {
 double matrix_cost(int tile, const Shape& s) { return num_tiles(tile, s) * item_cost(tile, s); }
 
 bool is_split(int prec) {
-  return prec == VBG_PREC_BF16X3 || prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3;
+  return prec == VBG_PREC_BF16X3 || prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3 ||
+         prec == VBG_PREC_BF16 || prec == VBG_PREC_BF16_OUT_BF16;
 }
 
-// The split-precision family is built on 32x32 MFMA blocks only (>= 64x64).
+// The split-precision and BF16 families are built on 32x32 MFMA blocks only
+// (>= 64x64).
 bool tile_allowed(int prec, int tile, unsigned mask) {
   if (mask && !(mask & (1u << tile))) return false;
   if (is_split(prec) && tile < T64) return false;
@@ -242,7 +244,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     err = "null dimension array";
     return VBG_ERR_INVALID_ARG;
   }
-  if (prec < VBG_PREC_FP32 || prec > VBG_PREC_FP32_SEQ) {
+  if (prec < VBG_PREC_FP32 || prec > VBG_PREC_BF16_OUT_BF16) {
     err = "unknown precision";
     return VBG_ERR_INVALID_ARG;
   }
@@ -258,6 +260,10 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     if (lda[b] < std::max(1, rows_a) || ldb[b] < std::max(1, rows_b) || ldc[b] < std::max(1, M[b])) {
       err = "leading dimension too small at matrix " + std::to_string(b);
       return VBG_ERR_INVALID_ARG;
+    }
+    if ((prec == VBG_PREC_BF16 || prec == VBG_PREC_BF16_OUT_BF16) && (lda[b] % 8 || ldb[b] % 8)) {
+      err = "BF16 operands need lda and ldb multiples of 8 (matrix " + std::to_string(b) + ")";
+      return VBG_ERR_NOT_SUPPORTED;
     }
     sh[b] = {M[b], N[b], K[b]};
     hp.desc[b] = {M[b], N[b], K[b], lda[b], ldb[b], ldc[b], 0, 1};
@@ -317,7 +323,10 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
 
   // 2. group matrices by kernel family/tile; 3. LPT over XCD queues.
   hp.nq = o.xcd_queues ? VBG_NUM_XCD : 1;
-  const int fam = prec == VBG_PREC_FP32_SEQ ? FAM_SEQ : (prec == VBG_PREC_FP32 ? FAM_FP32 : FAM_SPLIT);
+  const int fam = prec == VBG_PREC_FP32_SEQ ? FAM_SEQ
+                  : prec == VBG_PREC_FP32   ? FAM_FP32
+                  : (prec == VBG_PREC_BF16 || prec == VBG_PREC_BF16_OUT_BF16) ? FAM_BF16
+                                            : FAM_SPLIT;
   for (int t = 0; t < NTILES; ++t) {
     std::vector<int> mats;
     for (int b = 0; b < batch; ++b)

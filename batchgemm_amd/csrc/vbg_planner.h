@@ -15,7 +15,7 @@ namespace vbg {
 
 // Kernel family a tile id runs on for a precision (one persistent launch per
 // family used by a plan).
-enum Family { FAM_FP32 = 0, FAM_SEQ = 1, FAM_SPLIT = 2 };
+enum Family { FAM_FP32 = 0, FAM_SEQ = 1, FAM_SPLIT = 2, FAM_BF16 = 3 };
 
 struct Launch {
   int tile = 0;                 // tile id (T16 .. T128)

@@ -88,8 +88,8 @@ struct SplitTile {
     const float* __restrict__ B = a.B[it.mat];
     float* __restrict__ C = a.C[it.mat];
     const int m0 = it.tm * BM, n0 = it.tn * BN;
-    int kb, ke;
-    vbg_k_range<BK>(d.K, d.ksplit, it.kslice, kb, ke);
+    const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
+    const int kb = kr.kb, ke = kr.ke;
     const int nk = (ke - kb + BK - 1) / BK;
     float sA = 1.0f, sB = 1.0f, post = 1.0f;
     if (MODE != MODE_BF16X3) {

@@ -144,18 +144,16 @@ __device__ __forceinline__ void vbg_persistent_loop(const vbg_args& a, Body&& bo
 
 // k range of an item's slice: slices cover [0, K) in chunks that are whole
 // multiples of BK, the last one taking the remainder.
+struct KRange {
+  int kb, ke;
+};
 template <int BK>
-__device__ __forceinline__ void vbg_k_range(int K, int ksplit, int slice, int& kb, int& ke) {
-  if (ksplit <= 1) {
-    kb = 0;
-    ke = K;
-    return;
-  }
+__device__ __forceinline__ KRange vbg_k_range(int K, int ksplit, int slice) {
+  if (ksplit <= 1) return {0, K};
   const int ktiles = (K + BK - 1) / BK;
   const int per = (ktiles + ksplit - 1) / ksplit;
-  kb = slice * per * BK;
-  ke = min(K, kb + per * BK);
-  if (kb > K) kb = K;
+  const int kb = min(K, slice * per * BK);
+  return {kb, min(K, kb + per * BK)};
 }
 
 // Epilogue for a 4-row column segment: C[row..row+3, col] = alpha*v + beta*C.

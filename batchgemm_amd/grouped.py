@@ -1,0 +1,120 @@
+"""Grouped GEMM forward/backward over the C ABI, and its torch.autograd.Function.
+
+Groups are stacked along rows: A is [S, K] with S = sum(m_sizes), group g owns
+rows [off_g, off_g + m_g); B is [G, K, N]; C = A_g @ B_g is [S, N].  Backward:
+dA_g = dC_g @ B_g^T, dB_g = A_g^T @ dC_g (both computed by the library's
+grouped kernels on the same variable-size work map, FP32 accumulate).
+Operands are BF16, contiguous, 16-byte aligned; K and N multiples of 8.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from ._lib import check, default_opts, lib
+from .api import _stream_ptr
+
+_DT = {torch.float32: 0, torch.bfloat16: 1}
+
+
+class GroupedPlan:
+    def __init__(self, m_sizes: Sequence[int], K: int, N: int, **opts):
+        self.m_sizes = np.asarray(m_sizes, np.int32)
+        self.G, self.K, self.N = len(self.m_sizes), int(K), int(N)
+        self.S = int(self.m_sizes.sum())
+        o = default_opts(**opts)
+        p = ctypes.c_void_p()
+        m = np.ascontiguousarray(self.m_sizes)
+        check(lib().vbg_grouped_plan_create(ctypes.byref(p), self.G,
+                                            m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            self.K, self.N, ctypes.byref(o)),
+              "vbg_grouped_plan_create")
+        self._p = p
+
+    def _check(self, A, B):
+        if A.dtype != torch.bfloat16 or B.dtype != torch.bfloat16:
+            raise TypeError("grouped GEMM operands must be bfloat16")
+        if tuple(A.shape) != (self.S, self.K) or tuple(B.shape) != (self.G, self.K, self.N):
+            raise ValueError(f"expected A {(self.S, self.K)} and B {(self.G, self.K, self.N)}, "
+                             f"got {tuple(A.shape)} and {tuple(B.shape)}")
+        if not (A.is_contiguous() and B.is_contiguous()):
+            raise ValueError("operands must be contiguous")
+
+    def forward(self, A: torch.Tensor, B: torch.Tensor, out_dtype=torch.bfloat16,
+                out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        self._check(A, B)
+        if out is None:
+            out = torch.empty((self.S, self.N), dtype=out_dtype, device=A.device)
+        check(lib().vbg_grouped_gemm_fwd(self._p, A.data_ptr(), B.data_ptr(), out.data_ptr(),
+                                         _DT[out.dtype], _stream_ptr(stream)), "vbg_grouped_gemm_fwd")
+        return out
+
+    def backward(self, A: torch.Tensor, B: torch.Tensor, dC: torch.Tensor, grad_dtype=torch.bfloat16,
+                 need_dA=True, need_dB=True, stream=None):
+        self._check(A, B)
+        if dC.dtype != torch.bfloat16:
+            dC = dC.to(torch.bfloat16)  # dC is an MFMA operand: BF16 only
+        dC = dC.contiguous()
+        if tuple(dC.shape) != (self.S, self.N):
+            raise ValueError("dC shape mismatch")
+        dA = torch.empty((self.S, self.K), dtype=grad_dtype, device=A.device) if need_dA else None
+        dB = torch.empty((self.G, self.K, self.N), dtype=grad_dtype, device=A.device) if need_dB else None
+        check(lib().vbg_grouped_gemm_bwd(self._p, A.data_ptr(), B.data_ptr(), dC.data_ptr(), 1,
+                                         dA.data_ptr() if dA is not None else None,
+                                         dB.data_ptr() if dB is not None else None,
+                                         _DT[grad_dtype], _stream_ptr(stream)), "vbg_grouped_gemm_bwd")
+        return dA, dB
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().vbg_grouped_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_plans: dict = {}
+
+
+def grouped_plan(m_sizes, K: int, N: int) -> GroupedPlan:
+    key = (tuple(int(m) for m in m_sizes), int(K), int(N), torch.cuda.current_device())
+    p = _plans.get(key)
+    if p is None:
+        p = _plans[key] = GroupedPlan(m_sizes, K, N)
+    return p
+
+
+class GroupedGemmFunction(torch.autograd.Function):
+    """C = grouped(A, B); backward through the library's grouped kernels."""
+
+    @staticmethod
+    def forward(ctx, A, B, plan: GroupedPlan, out_dtype, grad_dtype):
+        ctx.save_for_backward(A, B)
+        ctx.plan = plan
+        ctx.grad_dtype = grad_dtype
+        return plan.forward(A.contiguous(), B.contiguous(), out_dtype=out_dtype)
+
+    @staticmethod
+    def backward(ctx, dC):
+        A, B = ctx.saved_tensors
+        dA, dB = ctx.plan.backward(A.contiguous(), B.contiguous(), dC, grad_dtype=ctx.grad_dtype,
+                                   need_dA=ctx.needs_input_grad[0], need_dB=ctx.needs_input_grad[1])
+        if dA is not None and dA.dtype != A.dtype:
+            dA = dA.to(A.dtype)
+        if dB is not None and dB.dtype != B.dtype:
+            dB = dB.to(B.dtype)
+        return dA, dB, None, None, None
+
+
+def grouped_gemm(A: torch.Tensor, B: torch.Tensor, m_sizes, out_dtype=torch.bfloat16,
+                 grad_dtype=torch.bfloat16) -> torch.Tensor:
+    """Differentiable grouped GEMM: rows of A grouped by `m_sizes`, times B[g]."""
+    plan = grouped_plan(m_sizes, A.shape[1], B.shape[2])
+    return GroupedGemmFunction.apply(A, B, plan, out_dtype, grad_dtype)

@@ -71,6 +71,75 @@ def cpu_baseline(M, N, K, alpha, beta, budget_s=12.0):
     }
 
 
+def rel_err_sample(w, vb, M, N, K, alpha, beta, C0, idx):
+    """Relative Frobenius error of C against FP64 (torch float64 on the GPU)
+    over sampled matrices, pooled."""
+    import torch
+    num = den = 0.0
+    for b in idx:
+        A = vb.a(b).double(); B = vb.b(b).double()
+        m, n = int(M[b]), int(N[b])
+        c0 = C0[int(vb.offC[b]): int(vb.offC[b]) + m * n].view(n, m).t().double()
+        ref = alpha * (A @ B) + beta * c0
+        num += float(torch.linalg.norm(vb.c(b).double() - ref) ** 2)
+        den += float(torch.linalg.norm(ref) ** 2)
+    return (num / den) ** 0.5
+
+
+def extras(bg, w, vb, M, N, K, alpha, beta, flops, fp32_value, args):
+    """bf16x3 on the same batch (GFLOP/s, ratio to FP32, error vs FP64 on a
+    sample), and the grouped BF16 forward+backward (G=128, K=2048, N=1408,
+    Zipf(1.2) group sizes summing to 32768)."""
+    import torch
+    out = {}
+    rng = np.random.default_rng(1)
+    big = np.where(M >= 512)[0]
+    idx = list(rng.choice(big, 4, replace=False)) + list(rng.choice(np.where(M < 512)[0], 28, replace=False))
+    for prec in ("bf16x3", "fp32"):
+        p = bg.Plan(M, N, K, prec=prec)
+        ms = time_launches(lambda: p.run(vb.dA, vb.dB, vb.dC, alpha, beta), args.steps, args.warmup)
+        C0 = vb.C.clone()
+        p.run(vb.dA, vb.dB, vb.dC, alpha, beta)
+        torch.cuda.synchronize()
+        err = rel_err_sample(w, vb, M, N, K, alpha, beta, C0, idx)
+        out[prec] = {"gflops": round(flops / (ms * 1e-3) / 1e9, 1), "ms": round(ms, 4),
+                     "rel_err_vs_fp64": float(f"{err:.3g}")}
+        log(f"[bench] {prec}: {ms:.3f} ms, {out[prec]['gflops']} GFLOP/s, rel err {err:.3g}")
+    out["bf16x3"]["speedup_vs_fp32"] = round(out["bf16x3"]["gflops"] / out["fp32"]["gflops"], 3)
+    # grouped forward + backward
+    from batchgemm_amd.grouped import GroupedPlan
+    m_sizes = w.zipf_groups(128, 32768, 1.2, seed=args.seed)
+    Kg, Ng = 2048, 1408
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    A = (torch.randn(32768, Kg, generator=g, device="cuda") * 0.02).to(torch.bfloat16)
+    B = (torch.randn(128, Kg, Ng, generator=g, device="cuda") * 0.02).to(torch.bfloat16)
+    dC = torch.randn(32768, Ng, generator=g, device="cuda").to(torch.bfloat16)
+    gp = GroupedPlan(m_sizes, Kg, Ng)
+    C = torch.empty(32768, Ng, dtype=torch.bfloat16, device="cuda")
+    dA = torch.empty_like(A); dB = torch.empty_like(B)
+    L = gp._p
+    lib = bg.lib()
+    from batchgemm_amd.api import _stream_ptr
+    def fwd():
+        bg._lib.check(lib.vbg_grouped_gemm_fwd(L, A.data_ptr(), B.data_ptr(), C.data_ptr(), 1,
+                                               _stream_ptr(None)), "grouped fwd")
+
+    def bwd():
+        bg._lib.check(lib.vbg_grouped_gemm_bwd(L, A.data_ptr(), B.data_ptr(), dC.data_ptr(), 1,
+                                               dA.data_ptr(), dB.data_ptr(), 1, _stream_ptr(None)),
+                      "grouped bwd")
+    f_ms = time_launches(fwd, args.steps, args.warmup)
+    b_ms = time_launches(bwd, args.steps, args.warmup)
+    gfl = 2.0 * 32768 * Kg * Ng
+    out["grouped_bf16"] = {
+        "workload": "G=128, K=2048, N=1408, Zipf(1.2) rows summing to 32768, BF16 in/out, FP32 accumulate",
+        "fwd_ms": round(f_ms, 4), "bwd_ms": round(b_ms, 4), "fwd_bwd_ms": round(f_ms + b_ms, 4),
+        "fwd_tflops": round(gfl / (f_ms * 1e-3) / 1e12, 1),
+        "bwd_tflops": round(2 * gfl / (b_ms * 1e-3) / 1e12, 1)}
+    log(f"[bench] grouped fwd {f_ms:.3f} ms bwd {b_ms:.3f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +150,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--policy", default="auto")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the bf16x3 and grouped fwd+bwd measurements")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -150,6 +221,8 @@ def main():
                      "launch_ms": round(sub_ms, 4), "launch_gflop": round(sub_flops / 1e9, 3)},
         "cpu_baseline": None,
     }
+    if args.prec == "fp32" and not args.no_extras:
+        line["extras"] = extras(bg, w, vb, M, N, K, alpha, beta, flops, value, args)
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         line["cpu_baseline"] = cpu_baseline(M, N, K, alpha, beta, args.cpu_seconds)

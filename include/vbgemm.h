@@ -39,15 +39,19 @@ typedef enum vbg_status {
   VBG_ERR_UNDEFINED = -6      /* the reference planner's behaviour is undefined here */
 } vbg_status;
 
-/* Arithmetic of a plan.  All paths take FP32 operands and write FP32 C. */
+/* Arithmetic of a plan.  Paths 0-4 take FP32 operands and write FP32 C. */
 typedef enum vbg_prec {
   VBG_PREC_FP32 = 0,        /* FP32 MFMA (v_mfma_f32_32x32x2_f32), FP32 accumulate */
   VBG_PREC_BF16X3 = 1,      /* x = hi + lo in bf16; hi*hi + hi*lo + lo*hi on bf16 MFMA */
   VBG_PREC_FP16_SCALED = 2, /* per-matrix power-of-two scale into FP16, FP16 MFMA */
   VBG_PREC_FP8_E4M3 = 3,    /* per-matrix amax scale (448/amax) into OCP e4m3, FP8 MFMA */
-  VBG_PREC_FP32_SEQ = 4     /* one sequential fma chain per element over ascending k
+  VBG_PREC_FP32_SEQ = 4,    /* one sequential fma chain per element over ascending k
                                (VALU); with alpha = beta = 1 it is seeded from C and is
                                bit-identical to the reference's correct tiles */
+  VBG_PREC_BF16 = 5,        /* BF16 operands (the pointer arrays hold BF16 data), FP32 C,
+                               v_mfma_f32_32x32x16_bf16; lda/ldb multiples of 8 and
+                               16-byte-aligned operand pointers */
+  VBG_PREC_BF16_OUT_BF16 = 6 /* as VBG_PREC_BF16 with a BF16 C */
 } vbg_prec;
 
 /* How the planner picks each matrix's tile and orders the work map. */

@@ -1,0 +1,165 @@
+"""Grouped GEMM forward/backward on the GPU (needs an MI355X).
+
+Forward and both gradients are checked against FP64 products of the same BF16
+values (<= 1e-5 relative with FP32 outputs, which is the FP32-accumulation
+bound; BF16 outputs add one BF16 rounding, <= 4e-3), against FP64 central
+finite differences of L = sum(C * dC) on a 4-group subset (M_g <= 64,
+K = N = 128), and against a looped torch.matmul."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(m_sizes, K, N, seed=0, std=0.02):
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    S = int(sum(m_sizes)); G = len(m_sizes)
+    A = (torch.randn(S, K, generator=g, device="cuda") * std).to(torch.bfloat16)
+    B = (torch.randn(G, K, N, generator=g, device="cuda") * std).to(torch.bfloat16)
+    dC = torch.randn(S, N, generator=g, device="cuda").to(torch.bfloat16)
+    return A, B, dC
+
+
+def ref64(A, B, dC, m_sizes):
+    A = A.double().cpu().numpy(); B = B.double().cpu().numpy(); dC = dC.double().cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(m_sizes)])
+    C = np.zeros((A.shape[0], B.shape[2])); dA = np.zeros_like(A); dB = np.zeros_like(B)
+    for g in range(len(m_sizes)):
+        s, e = off[g], off[g + 1]
+        C[s:e] = A[s:e] @ B[g]
+        dA[s:e] = dC[s:e] @ B[g].T
+        dB[g] = A[s:e].T @ dC[s:e]
+    return C, dA, dB
+
+
+def rel(x, r):
+    x = np.asarray(x, np.float64); d = np.linalg.norm(r)
+    return float(np.linalg.norm(x - r) / (d if d > 0 else 1.0))
+
+
+@pytest.mark.parametrize("m_sizes", [[5, 0, 64, 1, 130, 33, 200, 7], [64, 17, 3, 48]])
+@pytest.mark.parametrize("KN", [(128, 128), (256, 72), (64, 320)])
+def test_grouped_fwd_bwd_vs_fp64(bg, gpu, m_sizes, KN):
+    import torch
+    from batchgemm_amd.grouped import GroupedPlan
+    K, N = KN
+    say(f"[grouped] m={m_sizes} K={K} N={N}")
+    A, B, dC = make(m_sizes, K, N, seed=len(m_sizes) + K)
+    p = GroupedPlan(m_sizes, K, N)
+    C32 = p.forward(A, B, out_dtype=torch.float32)
+    C16 = p.forward(A, B, out_dtype=torch.bfloat16)
+    dA, dB = p.backward(A, B, dC, grad_dtype=torch.float32)
+    dA16, dB16 = p.backward(A, B, dC, grad_dtype=torch.bfloat16)
+    torch.cuda.synchronize()
+    C, rA, rB = ref64(A, B, dC, m_sizes)
+    assert rel(C32.cpu().numpy(), C) <= 1e-5
+    assert rel(dA.cpu().numpy(), rA) <= 1e-5
+    assert rel(dB.cpu().numpy(), rB) <= 1e-5
+    assert rel(C16.float().cpu().numpy(), C) <= 4e-3
+    assert rel(dA16.float().cpu().numpy(), rA) <= 4e-3
+    assert rel(dB16.float().cpu().numpy(), rB) <= 4e-3
+    for g, m in enumerate(m_sizes):  # empty groups: exact zero gradient
+        if m == 0:
+            assert torch.count_nonzero(dB[g]) == 0
+
+
+def test_grouped_finite_differences_fp64(bg, gpu):
+    """Directional FP64 central differences of L(A, B) = sum(C * dC) on a
+    4-group subset: L is bilinear, so the difference quotient is the exact
+    directional derivative up to FP64 rounding."""
+    import torch
+    from batchgemm_amd.grouped import GroupedPlan
+    say("[grouped] finite differences")
+    m_sizes = [64, 1, 37, 20]
+    K = N = 128
+    A, B, dC = make(m_sizes, K, N, seed=3)
+    dA, dB = GroupedPlan(m_sizes, K, N).backward(A, B, dC, grad_dtype=torch.float32)
+    torch.cuda.synchronize()
+    A64, B64, dC64 = (t.double().cpu().numpy() for t in (A, B, dC))
+    off = np.concatenate([[0], np.cumsum(m_sizes)])
+
+    def L(a, b):
+        return sum(float(np.sum((a[off[g]:off[g + 1]] @ b[g]) * dC64[off[g]:off[g + 1]]))
+                   for g in range(len(m_sizes)))
+
+    rng = np.random.default_rng(0)
+    eps = 1e-3
+    for _ in range(4):
+        VA = rng.standard_normal(A64.shape); VB = rng.standard_normal(B64.shape)
+        fd_a = (L(A64 + eps * VA, B64) - L(A64 - eps * VA, B64)) / (2 * eps)
+        fd_b = (L(A64, B64 + eps * VB) - L(A64, B64 - eps * VB)) / (2 * eps)
+        ga = dA.double().cpu().numpy(); gb = dB.double().cpu().numpy()
+        got_a = float(np.sum(ga * VA)); got_b = float(np.sum(gb * VB))
+        # <E, V> for a gradient error E and a unit-variance random direction V
+        # has standard deviation ||E||_F: allow 4 sigma of a 1e-5 relative error
+        assert abs(got_a - fd_a) <= 4e-5 * np.linalg.norm(ga), (got_a, fd_a)
+        assert abs(got_b - fd_b) <= 4e-5 * np.linalg.norm(gb), (got_b, fd_b)
+        assert abs(fd_a) > 1e-3 * np.linalg.norm(ga)  # the direction is not degenerate
+
+
+def test_grouped_autograd_vs_looped_matmul(bg, gpu):
+    import torch
+    from batchgemm_amd.grouped import grouped_gemm
+    say("[grouped] autograd vs looped matmul")
+    m_sizes = [96, 0, 31, 128, 5]
+    K, N = 192, 136
+    A, B, dC = make(m_sizes, K, N, seed=7)
+    a = A.clone().requires_grad_(True); b = B.clone().requires_grad_(True)
+    C = grouped_gemm(a, b, m_sizes, out_dtype=torch.float32, grad_dtype=torch.float32)
+    C.backward(dC.float())
+    # looped torch.matmul in float32 on the same BF16 values
+    a2 = A.float().requires_grad_(True); b2 = B.float().requires_grad_(True)
+    outs, s = [], 0
+    for g, m in enumerate(m_sizes):
+        outs.append(a2[s:s + m] @ b2[g]); s += m
+    C2 = torch.cat(outs)
+    C2.backward(dC.float())
+    # C is FP32; the gradients come back in the inputs' dtype (BF16), so they
+    # carry one BF16 rounding (2^-9 relative) on top of FP32 accumulation
+    assert rel(C.detach().float().cpu().numpy(), C2.detach().double().cpu().numpy()) <= 1e-5
+    for x, y in [(a.grad, a2.grad), (b.grad, b2.grad)]:
+        assert x.dtype == torch.bfloat16
+        assert rel(x.float().cpu().numpy(), y.double().cpu().numpy()) <= 4e-3
+        want = y.to(torch.bfloat16).float()
+        assert rel(x.float().cpu().numpy(), want.double().cpu().numpy()) <= 4e-3
+
+
+def test_grouped_config4_shape_properties(bg, gpu):
+    """G=128, K=2048, N=1408, Zipf(1.2) group sizes summing to 32768: random
+    probes C_g x, dA_g y, dB_g x against FP64 products of the same BF16 values
+    evaluated right to left (a size-independent check over whole outputs)."""
+    import torch
+    from batchgemm_amd import workloads as w
+    from batchgemm_amd.grouped import GroupedPlan
+    say("[grouped] config4")
+    m_sizes = w.zipf_groups(128, 32768, 1.2, seed=0)
+    assert m_sizes.sum() == 32768 and m_sizes.min() >= 1
+    K, N = 2048, 1408
+    A, B, dC = make(m_sizes, K, N, seed=0)
+    p = GroupedPlan(m_sizes, K, N)
+    C = p.forward(A, B, out_dtype=torch.float32)
+    dA, dB = p.backward(A, B, dC, grad_dtype=torch.float32)
+    torch.cuda.synchronize()
+    off = np.concatenate([[0], np.cumsum(m_sizes)])
+    A64, B64, dC64 = A.double(), B.double(), dC.double()
+    x = torch.randn(N, dtype=torch.float64, device="cuda")
+    y = torch.randn(K, dtype=torch.float64, device="cuda")
+    for g in range(0, 128, 9):
+        s, e = int(off[g]), int(off[g + 1])
+        # C_g x = A_g (B_g x); dA_g y = dC_g (B_g^T y); dB_g x = A_g^T (dC_g x)
+        r1 = A64[s:e] @ (B64[g] @ x); g1 = C[s:e].double() @ x
+        r2 = dC64[s:e] @ (B64[g].T @ y); g2 = dA[s:e].double() @ y
+        r3 = A64[s:e].T @ (dC64[s:e] @ x); g3 = dB[g].double() @ x
+        for gg, rr in [(g1, r1), (g2, r2), (g3, r3)]:
+            # FP32 accumulation over reductions of K=2048 (C, dA) and up to
+            # ~2e4 rows (dB of the largest groups): sqrt(n)*2^-24 ~ 1e-5, so
+            # the bar is 5e-5
+            assert float(torch.linalg.norm(gg - rr) / torch.linalg.norm(rr)) <= 5e-5, g
