@@ -23,7 +23,7 @@ VBG_ERR_ALLOC = -4
 VBG_ERR_PLAN_MISMATCH = -5
 VBG_ERR_UNDEFINED = -6
 
-PREC = {"fp32": 0, "bf16x3": 1, "fp16": 2, "fp8": 3, "fp32_seq": 4}
+PREC = {"fp32": 0, "bf16x3": 1, "fp16": 2, "fp8": 3, "fp32_seq": 4, "bf16": 5, "bf16_out_bf16": 6}
 POLICY = {"auto": 0, "reference": 1, "fixed": 2, "wang": 3, "li": 4}
 TILES = {"16x16": 0, "32x32": 1, "64x64": 2, "128x64": 3, "64x128": 4, "128x128": 5}
 TILE_SHAPES = [(16, 16), (32, 32), (64, 64), (128, 64), (64, 128), (128, 128)]
@@ -56,6 +56,8 @@ class vbg_opts(ctypes.Structure):
         ("xcd_queues", ctypes.c_int),
         ("host_only", ctypes.c_int),
         ("num_cus", ctypes.c_int),
+        ("force_ksplit", ctypes.c_int),
+        ("tile_group", ctypes.c_int),
     ]
 
 

@@ -50,6 +50,9 @@ struct vbg_plan {
   float* d_scales = nullptr;  // [4 * batch] (FP16 / FP8 paths)
   unsigned* d_amax = nullptr; // [2 * batch] float bits of per-matrix amax
   vbg_chunk* d_chunks = nullptr;
+  vbg_chunk* d_red = nullptr;     // split-K reduction chunks
+  int64_t* d_ws_off = nullptr;    // per-matrix workspace offsets (floats)
+  float* d_ws = nullptr;          // split-K partial panels (own allocation)
   struct DevLaunch {
     vbg_item* items;
     int* q_begin;
@@ -147,6 +150,10 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     off = align256(off + sizeof(unsigned) * 2 * std::max(1, hp.batch));
     const size_t off_chunks = off;
     off = align256(off + sizeof(vbg_chunk) * hp.chunks.size());
+    const size_t off_red = off;
+    off = align256(off + sizeof(vbg_chunk) * hp.red_chunks.size());
+    const size_t off_wsoff = off;
+    off = align256(off + sizeof(int64_t) * std::max(1, hp.batch));
     std::vector<size_t> oi, oq, oc;
     for (const auto& L : hp.launches) {
       oi.push_back(off);
@@ -160,6 +167,9 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     if (hp.batch) std::memcpy(host.data() + off_desc, hp.desc.data(), sizeof(vbg_mat_desc) * hp.batch);
     if (!hp.chunks.empty())
       std::memcpy(host.data() + off_chunks, hp.chunks.data(), sizeof(vbg_chunk) * hp.chunks.size());
+    if (!hp.red_chunks.empty())
+      std::memcpy(host.data() + off_red, hp.red_chunks.data(), sizeof(vbg_chunk) * hp.red_chunks.size());
+    if (hp.batch) std::memcpy(host.data() + off_wsoff, hp.ws_off.data(), sizeof(int64_t) * hp.batch);
     for (size_t l = 0; l < hp.launches.size(); ++l) {
       const auto& L = hp.launches[l];
       std::memcpy(host.data() + oi[l], L.items.data(), sizeof(vbg_item) * L.items.size());
@@ -181,6 +191,16 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     p->d_scales = (float*)(base + off_scales);
     p->d_amax = (unsigned*)(base + off_amax);
     p->d_chunks = (vbg_chunk*)(base + off_chunks);
+    p->d_red = (vbg_chunk*)(base + off_red);
+    p->d_ws_off = (int64_t*)(base + off_wsoff);
+    if (hp.ws_floats > 0) {
+      e = hipMalloc(&p->d_ws, sizeof(float) * hp.ws_floats);
+      if (e != hipSuccess) {
+        (void)hipFree(p->dbuf);
+        delete p;
+        return hip_fail(e, "hipMalloc(split-K workspace)");
+      }
+    }
     for (size_t l = 0; l < hp.launches.size(); ++l)
       p->dl.push_back({(vbg_item*)(base + oi[l]), (int*)(base + oq[l]), (unsigned*)(base + oc[l])});
   }
@@ -194,6 +214,10 @@ int vbg_plan_destroy(vbg_plan* p) {
   if (p->dbuf) {
     hipError_t e = hipFree(p->dbuf);
     if (e != hipSuccess) rc = hip_fail(e, "hipFree(plan)");
+  }
+  if (p->d_ws) {
+    hipError_t e = hipFree(p->d_ws);
+    if (e != hipSuccess) rc = hip_fail(e, "hipFree(workspace)");
   }
   delete p;
   return rc;
@@ -227,8 +251,8 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
     a.A = dA;
     a.B = dB;
     a.C = dC;
-    a.ws = nullptr;
-    a.ws_off = nullptr;
+    a.ws = p->d_ws;
+    a.ws_off = p->d_ws_off;
     a.scales = p->d_scales;
     a.alpha = alpha;
     a.beta = beta;
@@ -246,6 +270,12 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
     }
     if (r == -1) return fail(VBG_ERR_NOT_SUPPORTED, "tile/precision combination not built");
     if (r != 0) return fail(VBG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(hipGetLastError()));
+  }
+  if (!hp.red_chunks.empty()) {
+    int r = vbg_launch_splitk_reduce(p->d_red, (int)hp.red_chunks.size(), p->d_desc, p->d_ws,
+                                     p->d_ws_off, dC, alpha, beta,
+                                     hp.prec == VBG_PREC_BF16_OUT_BF16, s);
+    if (r != 0) return fail(VBG_ERR_HIP, std::string("split-K reduction launch: ") + hipGetErrorString(hipGetLastError()));
   }
   return VBG_OK;
 }

@@ -18,6 +18,16 @@ typedef unsigned short u16;
 
 __device__ __forceinline__ u16 ld16(const u16* p) { return *p; }
 
+// LDS row layout shared by loader and tile: rows of BK BF16 (BK*2 bytes) in
+// 16-byte units; unit u of row r sits at unit u ^ ((r / 2) % 8): rows are 128
+// bytes, so rows 2j and 2j+1 share a 256-byte bank line, and any 16
+// consecutive rows read at one logical unit land in 16 distinct 16-byte bank
+// slots.  No row padding.
+template <int PITCH>
+__device__ __forceinline__ int bswz(int row, int byte) {
+  return row * PITCH + ((((byte >> 4) ^ (row >> 1)) & 7) << 4) + (byte & 15);
+}
+
 // BF16 operand loader.  LDS plane layout [mn][k], k contiguous, PITCH bytes.
 // KC (k contiguous in HBM): a thread loads 8 consecutive k of one row.
 // MC (mn contiguous in HBM): a thread loads 8 consecutive mn at k and at k+1,
@@ -98,15 +108,15 @@ struct Bf16Loader {
       int row, kk;
       coords(v, row, kk);
       if (KC) {
-        *(v4u*)(plane + row * PITCH + 2 * kk) = r[v];
+        *(v4u*)(plane + bswz<PITCH>(row, 2 * kk)) = r[v];
       } else {
         const v4u a = r[2 * v], b = r[2 * v + 1];  // k, k+1; 8 mn each
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const u32 even = (a[e] & 0xffffu) | (b[e] << 16);
           const u32 odd = (a[e] >> 16) | (b[e] & 0xffff0000u);
-          *(u32*)(plane + (row + 2 * e) * PITCH + 2 * kk) = even;
-          *(u32*)(plane + (row + 2 * e + 1) * PITCH + 2 * kk) = odd;
+          *(u32*)(plane + bswz<PITCH>(row + 2 * e, 2 * kk)) = even;
+          *(u32*)(plane + bswz<PITCH>(row + 2 * e + 1, 2 * kk)) = odd;
         }
       }
     }
@@ -116,10 +126,26 @@ struct Bf16Loader {
 __device__ __forceinline__ float bf2f(u16 x) { return __uint_as_float((u32)x << 16); }
 
 // 4-row column segment of a column-major BF16 C: round alpha*v + beta*C to BF16.
+typedef u16 u16x4 __attribute__((ext_vector_type(4), aligned(2)));
+
+__device__ __forceinline__ u16 f2bf(float x) {
+  const __bf16 h = (__bf16)x;
+  return *(const u16*)&h;
+}
+
 __device__ __forceinline__ void store4_bf16(u16* C, int ldc, int row, int col, int M, int N, v4f v,
                                             float alpha, float beta) {
   if (col >= N || row >= M) return;
   u16* c = C + (size_t)col * ldc + row;
+  if (row + 3 < M) {  // 4 consecutive rows of a column: one 8-byte access
+    v4f o = v * alpha;
+    if (beta != 0.0f) {
+      const u16x4 old = *(const u16x4*)c;
+      o += beta * v4f{bf2f(old[0]), bf2f(old[1]), bf2f(old[2]), bf2f(old[3])};
+    }
+    *(u16x4*)c = u16x4{f2bf(o[0]), f2bf(o[1]), f2bf(o[2]), f2bf(o[3])};
+    return;
+  }
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     if (row + e < M) {
@@ -134,7 +160,8 @@ __device__ __forceinline__ void store4_bf16(u16* C, int ldc, int row, int col, i
 template <int BM, int BN, bool TA, bool TB, bool OUT_BF16>
 struct Bf16Tile {
   static constexpr int BK = 64;
-  static constexpr int PITCH = BK * 2 + 16;
+  static constexpr int PITCH = BK * 2;  // 128 bytes: 8 swizzled 16-byte units
+  static_assert(PITCH == 128, "bswz assumes 8 units per row");
   static constexpr int PLANE_A = BM * PITCH, PLANE_B = BN * PITCH;
   static constexpr int STAGE = PLANE_A + PLANE_B;
   static constexpr int LDS_FLOATS = 2 * STAGE / 4;
@@ -146,9 +173,9 @@ struct Bf16Tile {
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds_f) {
     char* lds = (char*)lds_f;
-    const vbg_mat_desc d = a.desc[it.mat];
-    const u16* __restrict__ A = (const u16*)a.A[it.mat];
-    const u16* __restrict__ B = (const u16*)a.B[it.mat];
+    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
+    const u16* __restrict__ A = (const u16*)vbg_uptr(a.A[it.mat]);
+    const u16* __restrict__ B = (const u16*)vbg_uptr(a.B[it.mat]);
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;
@@ -191,9 +218,9 @@ struct Bf16Tile {
         const int koff = (h * (BK / 2) + 8 * s) * 2;
         v8bf av[TM], bv[TN];
 #pragma unroll
-        for (int r = 0; r < TM; ++r) av[r] = *(const v8bf*)(As + (wr * WTM + r * 32 + l32) * PITCH + koff);
+        for (int r = 0; r < TM; ++r) av[r] = *(const v8bf*)(As + bswz<PITCH>(wr * WTM + r * 32 + l32, koff));
 #pragma unroll
-        for (int c = 0; c < TN; ++c) bv[c] = *(const v8bf*)(Bs + (wc * WTN + c * 32 + l32) * PITCH + koff);
+        for (int c = 0; c < TN; ++c) bv[c] = *(const v8bf*)(Bs + bswz<PITCH>(wc * WTN + c * 32 + l32, koff));
 #pragma unroll
         for (int r = 0; r < TM; ++r)
 #pragma unroll
@@ -207,6 +234,7 @@ struct Bf16Tile {
       }
       __syncthreads();
     }
+    const OutTile o = vbg_out(a, d, it, 1.0f);
 #pragma unroll
     for (int r = 0; r < TM; ++r)
 #pragma unroll
@@ -216,10 +244,10 @@ struct Bf16Tile {
           const int row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
           const int col = n0 + wc * WTN + c * 32 + l32;
           v4f v = {acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
-          if (OUT_BF16)
-            store4_bf16((u16*)a.C[it.mat], d.ldc, row, col, d.M, d.N, v, a.alpha, a.beta);
+          if (OUT_BF16 && !o.partial)
+            store4_bf16((u16*)o.p, o.ld, row, col, d.M, d.N, v, o.alpha, o.beta);
           else
-            vbg_store4(a.C[it.mat], d.ldc, row, col, d.M, d.N, v, a.alpha, a.beta);
+            vbg_store4(o.p, o.ld, row, col, d.M, d.N, v, o.alpha, o.beta);
         }
   }
 };

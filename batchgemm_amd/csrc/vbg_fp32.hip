@@ -37,10 +37,9 @@ struct Fp32MfmaTile {
   using LB = TileLoader<BN, BK, P, !TB>;
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
-    const vbg_mat_desc d = a.desc[it.mat];
-    const float* __restrict__ A = a.A[it.mat];
-    const float* __restrict__ B = a.B[it.mat];
-    float* __restrict__ C = a.C[it.mat];
+    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
+    const float* __restrict__ A = vbg_uptr(a.A[it.mat]);
+    const float* __restrict__ B = vbg_uptr(a.B[it.mat]);
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;
@@ -60,12 +59,15 @@ struct Fp32MfmaTile {
 
     LA la;
     LB lb;
+    la.init(A, d.lda, m0, d.M, d.K);
+    lb.init(B, d.ldb, n0, d.N, d.K);
+    typename LA::Regs ra;
+    typename LB::Regs rb;
     if (nk > 0) {
-      const bool tail0 = kb + BK > ke;
-      la.load(A, d.lda, m0, kb, d.M, ke, tail0);
-      lb.load(B, d.ldb, n0, kb, d.N, ke, tail0);
-      la.store(lds);
-      lb.store(lds + BM * P);
+      la.fetch(ra, kb, ke);
+      lb.fetch(rb, kb, ke);
+      LA::store(ra, lds);
+      LB::store(rb, lds + BM * P);
     }
     __syncthreads();
 
@@ -74,9 +76,8 @@ struct Fp32MfmaTile {
       const bool more = kt + 1 < nk;
       if (more) {
         const int k0 = kb + (kt + 1) * BK;
-        const bool tail = k0 + BK > ke;
-        la.load(A, d.lda, m0, k0, d.M, ke, tail);
-        lb.load(B, d.ldb, n0, k0, d.N, ke, tail);
+        la.fetch(ra, k0, ke);
+        lb.fetch(rb, k0, ke);
       }
       const float* As = lds + cur * STAGE;
       const float* Bs = As + BM * P;
@@ -99,14 +100,15 @@ struct Fp32MfmaTile {
       }
       if (more) {
         float* nxt = lds + (cur ^ 1) * STAGE;
-        la.store(nxt);
-        lb.store(nxt + BM * P);
+        LA::store(ra, nxt);
+        LB::store(rb, nxt + BM * P);
       }
       __syncthreads();
     }
 
     // 32x32x2 f32 accumulator layout: register 4g+e of lane l holds
     // row 8g + 4*(l/32) + e, column l%32 of the 32x32 block.
+    const OutTile o = vbg_out(a, d, it, 1.0f);
 #pragma unroll
     for (int r = 0; r < TM; ++r)
 #pragma unroll
@@ -116,7 +118,7 @@ struct Fp32MfmaTile {
           const int row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
           const int col = n0 + wc * WTN + s * 32 + l32;
           v4f v = {acc[r][s][4 * g], acc[r][s][4 * g + 1], acc[r][s][4 * g + 2], acc[r][s][4 * g + 3]};
-          vbg_store4(C, d.ldc, row, col, d.M, d.N, v, a.alpha, a.beta);
+          vbg_store4(o.p, o.ld, row, col, d.M, d.N, v, o.alpha, o.beta);
         }
   }
 };
@@ -144,10 +146,9 @@ struct Fp32SmallTile {
   using LB = TileLoader<BMN, BK, P, !TB>;
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
-    const vbg_mat_desc d = a.desc[it.mat];
-    const float* __restrict__ A = a.A[it.mat];
-    const float* __restrict__ B = a.B[it.mat];
-    float* __restrict__ C = a.C[it.mat];
+    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
+    const float* __restrict__ A = vbg_uptr(a.A[it.mat]);
+    const float* __restrict__ B = vbg_uptr(a.B[it.mat]);
     const int m0 = it.tm * BMN, n0 = it.tn * BMN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;
@@ -161,12 +162,15 @@ struct Fp32SmallTile {
     v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
     LA la;
     LB lb;
+    la.init(A, d.lda, m0, d.M, d.K);
+    lb.init(B, d.ldb, n0, d.N, d.K);
+    typename LA::Regs ra;
+    typename LB::Regs rb;
     if (nk > 0) {
-      const bool tail0 = kb + BK > ke;
-      la.load(A, d.lda, m0, kb, d.M, ke, tail0);
-      lb.load(B, d.ldb, n0, kb, d.N, ke, tail0);
-      la.store(lds);
-      lb.store(lds + BMN * P);
+      la.fetch(ra, kb, ke);
+      lb.fetch(rb, kb, ke);
+      LA::store(ra, lds);
+      LB::store(rb, lds + BMN * P);
     }
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
@@ -174,9 +178,8 @@ struct Fp32SmallTile {
       const bool more = kt + 1 < nk;
       if (more) {
         const int k0 = kb + (kt + 1) * BK;
-        const bool tail = k0 + BK > ke;
-        la.load(A, d.lda, m0, k0, d.M, ke, tail);
-        lb.load(B, d.ldb, n0, k0, d.N, ke, tail);
+        la.fetch(ra, k0, ke);
+        lb.fetch(rb, k0, ke);
       }
       const float* As = lds + cur * STAGE;
       const float* Bs = As + BMN * P;
@@ -189,12 +192,13 @@ struct Fp32SmallTile {
       }
       if (more) {
         float* nxt = lds + (cur ^ 1) * STAGE;
-        la.store(nxt);
-        lb.store(nxt + BMN * P);
+        LA::store(ra, nxt);
+        LB::store(rb, nxt + BMN * P);
       }
       __syncthreads();
     }
     const int row0 = 4 * g;  // rows row0..row0+3 of the 16x16 block, column l16
+    const OutTile o = vbg_out(a, d, it, 1.0f);
     if (KSPLIT) {
       *(v4f*)(lds + wave * 256 + lane * 4) = acc;
       __syncthreads();
@@ -202,12 +206,12 @@ struct Fp32SmallTile {
         v4f s = acc;
 #pragma unroll
         for (int w = 1; w < 4; ++w) s += *(const v4f*)(lds + w * 256 + lane * 4);
-        vbg_store4(C, d.ldc, m0 + row0, n0 + l16, d.M, d.N, s, a.alpha, a.beta);
+        vbg_store4(o.p, o.ld, m0 + row0, n0 + l16, d.M, d.N, s, o.alpha, o.beta);
       }
       __syncthreads();
     } else {
-      vbg_store4(C, d.ldc, m0 + (wave & 1) * 16 + row0, n0 + (wave >> 1) * 16 + l16, d.M, d.N, acc,
-                 a.alpha, a.beta);
+      vbg_store4(o.p, o.ld, m0 + (wave & 1) * 16 + row0, n0 + (wave >> 1) * 16 + l16, d.M, d.N, acc,
+                 o.alpha, o.beta);
     }
   }
 };
@@ -231,10 +235,10 @@ struct SeqTile {
   }
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
-    const vbg_mat_desc d = a.desc[it.mat];
-    const float* A = a.A[it.mat];
-    const float* B = a.B[it.mat];
-    float* C = a.C[it.mat];
+    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
+    const float* A = vbg_uptr(a.A[it.mat]);
+    const float* B = vbg_uptr(a.B[it.mat]);
+    float* C = vbg_uptr(a.C[it.mat]);
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     float* As = lds;
     float* Bs = lds + BK * PA;

@@ -21,3 +21,9 @@ int vbg_launch_scales(int prec, int batch, int ta, int tb, const vbg_mat_desc* d
 // BF16-operand family (grouped passes); a->prec selects FP32 or BF16 C.
 int vbg_launch_bf16(int tile, int ta, int tb, const vbg_args* a, int grid, void* stream);
 int vbg_occupancy_bf16(int prec, int tile, int ta, int tb);
+
+// Split-K reduction over the plan's reduction chunks (elements [begin, end)
+// of one split matrix's M x N output in column-major order).
+int vbg_launch_splitk_reduce(const vbg_chunk* chunks, int n_chunks, const vbg_mat_desc* desc,
+                             const float* ws, const int64_t* ws_off, float* const* C, float alpha,
+                             float beta, int out_bf16, void* stream);

@@ -320,13 +320,53 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   if (is_split(prec))
     for (int b = 0; b < batch; ++b) tile[b] = std::max(tile[b], (int)T64);
   hp.tile_of.assign(batch, -1);
-
-  // 2. group matrices by kernel family/tile; 3. LPT over XCD queues.
-  hp.nq = o.xcd_queues ? VBG_NUM_XCD : 1;
   const int fam = prec == VBG_PREC_FP32_SEQ ? FAM_SEQ
                   : prec == VBG_PREC_FP32   ? FAM_FP32
                   : (prec == VBG_PREC_BF16 || prec == VBG_PREC_BF16_OUT_BF16) ? FAM_BF16
                                             : FAM_SPLIT;
+
+  // 1b. split-K.  An item longer than the mean load of a workgroup slot bounds
+  // the launch's makespan from below; when a matrix's items exceed 1.25x that
+  // mean, its K is cut into S slices (whole k-tiles each) of about half the
+  // mean, whose FP32 partial panels go to the workspace and are summed by the
+  // split-K reduction.  Never for FP32_SEQ (one sequential chain per element
+  // by definition).
+  hp.ws_off.assign(batch, -1);
+  if (prec != VBG_PREC_FP32_SEQ && o.split_k_min >= 0) {
+    const int kmin = o.split_k_min > 0 ? o.split_k_min : 256;
+    double total = 0.0;
+    for (int b = 0; b < batch; ++b)
+      if (M[b] > 0 && N[b] > 0) total += matrix_cost(tile[b], sh[b]);
+    const double avg = total / ((double)cus * occ(fam, T128));
+    const int64_t ws_cap = (int64_t)1 << 27;  // floats (512 MB)
+    int64_t ws = 0;
+    for (int b = 0; b < batch; ++b) {
+      if (M[b] == 0 || N[b] == 0) continue;
+      const int bk = fam == FAM_BF16 ? 64 : kTileBK[tile[b]];
+      const int ks = cdiv(K[b], bk);
+      int S = 1;
+      if (o.force_ksplit > 0) {
+        S = std::min(o.force_ksplit, ks);
+      } else if (K[b] >= kmin) {
+        const double ic = item_cost(tile[b], sh[b]);
+        if (ic > 1.25 * avg) S = std::min<int>({(int)std::ceil(2.0 * ic / avg), ks / 2, 16});
+      }
+      const int64_t need = (int64_t)S * M[b] * N[b];
+      if (S >= 2 && ws + need <= ws_cap) {
+        hp.desc[b].ksplit = S;
+        hp.ws_off[b] = ws;
+        ws += need;
+      }
+    }
+    hp.ws_floats = ws;
+  }
+  auto slice_cost = [&](int t, int b) {
+    const int S = hp.desc[b].ksplit;
+    return kItemFixed + (item_cost(t, sh[b]) - kItemFixed) / S;
+  };
+
+  // 2. group matrices by kernel family/tile; 3. LPT over XCD queues.
+  hp.nq = o.xcd_queues ? VBG_NUM_XCD : 1;
   for (int t = 0; t < NTILES; ++t) {
     std::vector<int> mats;
     for (int b = 0; b < batch; ++b)
@@ -355,12 +395,18 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     for (int q = 0; q < hp.nq; ++q) {
       // per-item cost decreasing; a matrix's tiles stay together
       std::stable_sort(qm[q].begin(), qm[q].end(), [&](int x, int y) {
-        return item_cost(t, sh[x]) > item_cost(t, sh[y]);
+        return slice_cost(t, x) > slice_cost(t, y);
       });
+      // tiles in runs of `grp` tile rows, column by column: the ~64 items an
+      // XCD runs at once form a grp x (64/grp) block of the matrix, touching
+      // grp A panels and 64/grp B panels instead of 64/tn rows x tn columns
+      const int grp = o.tile_group > 0 ? o.tile_group : 8;
       for (int b : qm[q]) {
         const int tm_n = cdiv(M[b], kTileBM[t]), tn_n = cdiv(N[b], kTileBN[t]);
-        for (int i = 0; i < tm_n; ++i)
-          for (int j = 0; j < tn_n; ++j) L.items.push_back({b, i, j, 0});
+        for (int s = 0; s < hp.desc[b].ksplit; ++s)
+          for (int i0 = 0; i0 < tm_n; i0 += grp)
+            for (int j = 0; j < tn_n; ++j)
+              for (int i = i0; i < std::min(tm_n, i0 + grp); ++i) L.items.push_back({b, i, j, s});
       }
       L.q_begin.push_back((int)L.items.size());
     }
@@ -372,7 +418,11 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   }
   std::stable_sort(hp.launches.begin(), hp.launches.end(),
                    [](const Launch& x, const Launch& y) { return x.cost > y.cost; });
-  hp.ws_off.assign(batch, -1);
+  for (int b = 0; b < batch; ++b) {
+    if (hp.desc[b].ksplit < 2) continue;
+    const int64_t n = (int64_t)M[b] * N[b], target = 1 << 16;
+    for (int64_t s0 = 0; s0 < n; s0 += target) hp.red_chunks.push_back({b, 0, s0, std::min(n, s0 + target)});
+  }
   if (prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3) {
     const int64_t target = 1 << 16;  // elements per chunk
     for (int b = 0; b < batch; ++b) {

@@ -37,6 +37,7 @@ struct HostPlan {
   int64_t ws_floats = 0;         // split-K workspace
   std::vector<int64_t> ws_off;   // per matrix (floats), -1 if not split
   std::vector<vbg_chunk> chunks; // amax pre-pass slices (FP16 / FP8)
+  std::vector<vbg_chunk> red_chunks;  // split-K reduction slices
 };
 
 // Fills `hp` from the shapes and options.  `occupancy(tile)` gives the

@@ -1,0 +1,46 @@
+// vbg_reduce.hip -- split-K reduction: C = alpha * sum_s W_s + beta * C for the
+// matrices whose K was split into d.ksplit slices.  Every slice wrote its FP32
+// partial panel W_s (column-major, ld = M) into the plan's workspace, so the
+// sum is a fixed-order loop: deterministic, no atomics, no memset.
+#include "vbg_launch.h"
+#include "vbg_tiles.h"
+
+namespace {
+
+__global__ void __launch_bounds__(VBG_THREADS)
+    vbg_splitk_reduce(const vbg_chunk* ch, const vbg_mat_desc* desc, const float* ws,
+                      const int64_t* ws_off, float* const* C, float alpha, float beta, int out_bf16) {
+  const vbg_chunk c = ch[blockIdx.x];
+  const vbg_mat_desc d = desc[c.mat];
+  const float* W = ws + ws_off[c.mat];
+  const size_t mn = (size_t)d.M * d.N;
+  for (int64_t e = c.begin + threadIdx.x; e < c.end; e += VBG_THREADS) {
+    const int i = (int)(e % d.M), j = (int)(e / d.M);
+    float s = 0.0f;
+    for (int k = 0; k < d.ksplit; ++k) s += W[k * mn + e];
+    const size_t ci = (size_t)j * d.ldc + i;
+    if (out_bf16) {
+      unsigned short* p = (unsigned short*)C[c.mat] + ci;
+      float o = alpha * s;
+      if (beta != 0.0f) o += beta * __uint_as_float((unsigned)*p << 16);
+      const __bf16 h = (__bf16)o;
+      *p = *(const unsigned short*)&h;
+    } else {
+      float* p = C[c.mat] + ci;
+      float o = alpha * s;
+      if (beta != 0.0f) o += beta * *p;
+      *p = o;
+    }
+  }
+}
+
+}  // namespace
+
+int vbg_launch_splitk_reduce(const vbg_chunk* chunks, int n_chunks, const vbg_mat_desc* desc,
+                             const float* ws, const int64_t* ws_off, float* const* C, float alpha,
+                             float beta, int out_bf16, void* stream) {
+  if (n_chunks <= 0) return 0;
+  hipLaunchKernelGGL(vbg_splitk_reduce, dim3(n_chunks), dim3(VBG_THREADS), 0, (hipStream_t)stream,
+                     chunks, desc, ws, ws_off, C, alpha, beta, out_bf16);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}

@@ -17,6 +17,16 @@ namespace {
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+
+// Round 4 floats to BF16 (nearest even) two at a time: the 2-wide form lowers
+// to one v_cvt_pk_bf16_f32 per pair.
+__device__ __forceinline__ v4bf bf16x4(v4f x) {
+  const v2bf a = __builtin_convertvector(v2f{x[0], x[1]}, v2bf);
+  const v2bf b = __builtin_convertvector(v2f{x[2], x[3]}, v2bf);
+  return __builtin_shufflevector(a, b, 0, 1, 2, 3);
+}
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef _Float16 v4h __attribute__((ext_vector_type(4)));
 
@@ -37,14 +47,22 @@ struct Mode<MODE_FP8> {
   static constexpr int EB = 1, PLANES = 1;
 };
 
-// LDS layout per operand plane: [mn][k] with k contiguous; pitch = BK*EB + 16
-// bytes.  Lane half h of an MFMA wave owns k in [h*BK/2, (h+1)*BK/2); step s
+// LDS layout per operand plane: [mn][k] with k contiguous, rows of BK*EB bytes
+// split into 4 units (16 B for BF16/FP16, 8 B for FP8: one MFMA operand read);
+// unit u of row r sits at unit u ^ ((r / RPB) % 4), RPB = 256 / row bytes, so
+// the 32 rows an MFMA operand read touches hit distinct LDS banks without row
+// padding.  Lane half h of an MFMA wave owns k in [h*BK/2, (h+1)*BK/2); step s
 // of the 32x32x16 MFMA takes 8 of them, k = h*BK/2 + 8s + [0, 8).
 template <int BM, int BN, int MODE, bool TA, bool TB>
 struct SplitTile {
   static constexpr int BK = 32;
   static constexpr int EB = Mode<MODE>::EB, PLANES = Mode<MODE>::PLANES;
-  static constexpr int PITCH = BK * EB + 16;  // bytes per LDS row
+  static constexpr int PITCH = BK * EB;  // bytes per LDS row (no padding: swizzled)
+  static constexpr int UNIT = PITCH / 4;  // bytes per MFMA operand read
+  static constexpr int RPB = 256 / PITCH;
+  __device__ __forceinline__ static int swz(int row, int byte) {
+    return row * PITCH + ((((byte / UNIT) ^ (row / RPB)) & 3) * UNIT) + (byte % UNIT);
+  }
   static constexpr int PLANE_A = BM * PITCH, PLANE_B = BN * PITCH;
   static constexpr int STAGE = PLANES * (PLANE_A + PLANE_B);  // bytes
   static constexpr int LDS_FLOATS = 2 * STAGE / 4;
@@ -56,17 +74,17 @@ struct SplitTile {
 
   // Quantise 4 consecutive k of one row and write them to the plane(s).
   template <class L>
-  __device__ __forceinline__ static void store(const L& ld, char* plane0, int plane_bytes, float s) {
+  __device__ __forceinline__ static void store(const typename L::Regs& rr, char* plane0,
+                                               int plane_bytes, float s) {
 #pragma unroll
     for (int v = 0; v < L::NV; ++v) {
       int row, kq;
       L::coords(v, row, kq);
-      const v4f x = ld.r[v];
-      char* dst = plane0 + row * PITCH + 4 * kq * EB;
+      const v4f x = rr[v];
+      char* dst = plane0 + swz(row, 4 * kq * EB);
       if (MODE == MODE_BF16X3) {
-        const v4bf hi = {(__bf16)x[0], (__bf16)x[1], (__bf16)x[2], (__bf16)x[3]};
-        const v4bf lo = {(__bf16)(x[0] - (float)hi[0]), (__bf16)(x[1] - (float)hi[1]),
-                         (__bf16)(x[2] - (float)hi[2]), (__bf16)(x[3] - (float)hi[3])};
+        const v4bf hi = bf16x4(x);
+        const v4bf lo = bf16x4(x - __builtin_convertvector(hi, v4f));  // x - hi is exact
         *(v4bf*)dst = hi;
         *(v4bf*)(dst + plane_bytes) = lo;
       } else if (MODE == MODE_FP16) {
@@ -83,19 +101,18 @@ struct SplitTile {
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds_f) {
     char* lds = (char*)lds_f;
-    const vbg_mat_desc d = a.desc[it.mat];
-    const float* __restrict__ A = a.A[it.mat];
-    const float* __restrict__ B = a.B[it.mat];
-    float* __restrict__ C = a.C[it.mat];
+    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
+    const float* __restrict__ A = vbg_uptr(a.A[it.mat]);
+    const float* __restrict__ B = vbg_uptr(a.B[it.mat]);
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;
     const int nk = (ke - kb + BK - 1) / BK;
     float sA = 1.0f, sB = 1.0f, post = 1.0f;
     if (MODE != MODE_BF16X3) {
-      sA = a.scales[4 * it.mat];
-      sB = a.scales[4 * it.mat + 1];
-      post = a.scales[4 * it.mat + 2];
+      sA = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, a.scales[4 * it.mat])));
+      sB = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, a.scales[4 * it.mat + 1])));
+      post = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, a.scales[4 * it.mat + 2])));
     }
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -110,29 +127,21 @@ struct SplitTile {
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[r][s][e] = 0.0f;
 
+    // Two register stages: while the MFMAs consume LDS buffer kt&1, tile kt+1
+    // (loaded from HBM one iteration earlier) is quantised into the other LDS
+    // buffer between the MFMA steps, and the loads of tile kt+2 are in flight.
     LA la;
     LB lb;
-    auto put = [&](char* stage) {
-      store(la, stage, PLANE_A, sA);
-      store(lb, stage + PLANES * PLANE_A, PLANE_B, sB);
+    la.init(A, d.lda, m0, d.M, d.K);
+    lb.init(B, d.ldb, n0, d.N, d.K);
+    typename LA::Regs la0, la1;
+    typename LB::Regs lb0, lb1;
+    auto load = [&](typename LA::Regs& ra, typename LB::Regs& rb, int kt) {
+      const int k0 = kb + kt * BK;
+      la.fetch(ra, k0, ke);
+      lb.fetch(rb, k0, ke);
     };
-    if (nk > 0) {
-      const bool tail0 = kb + BK > ke;
-      la.load(A, d.lda, m0, kb, d.M, ke, tail0);
-      lb.load(B, d.ldb, n0, kb, d.N, ke, tail0);
-      put(lds);
-    }
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-      const int cur = kt & 1;
-      const bool more = kt + 1 < nk;
-      if (more) {
-        const int k0 = kb + (kt + 1) * BK;
-        const bool tail = k0 + BK > ke;
-        la.load(A, d.lda, m0, k0, d.M, ke, tail);
-        lb.load(B, d.ldb, n0, k0, d.N, ke, tail);
-      }
-      const char* As = lds + cur * STAGE;
+    auto compute = [&](const char* As, auto&& mid) {
       const char* Bs = As + PLANES * PLANE_A;
 #pragma unroll
       for (int s = 0; s < BK / 16; ++s) {
@@ -140,9 +149,9 @@ struct SplitTile {
         if (MODE == MODE_FP8) {
           long av[TM], bv[TN];
 #pragma unroll
-          for (int r = 0; r < TM; ++r) av[r] = *(const long*)(As + (wr * WTM + r * 32 + l32) * PITCH + koff);
+          for (int r = 0; r < TM; ++r) av[r] = *(const long*)(As + swz(wr * WTM + r * 32 + l32, koff));
 #pragma unroll
-          for (int c = 0; c < TN; ++c) bv[c] = *(const long*)(Bs + (wc * WTN + c * 32 + l32) * PITCH + koff);
+          for (int c = 0; c < TN; ++c) bv[c] = *(const long*)(Bs + swz(wc * WTN + c * 32 + l32, koff));
 #pragma unroll
           for (int r = 0; r < TM; ++r)
 #pragma unroll
@@ -151,9 +160,9 @@ struct SplitTile {
         } else if (MODE == MODE_FP16) {
           v8h av[TM], bv[TN];
 #pragma unroll
-          for (int r = 0; r < TM; ++r) av[r] = *(const v8h*)(As + (wr * WTM + r * 32 + l32) * PITCH + koff);
+          for (int r = 0; r < TM; ++r) av[r] = *(const v8h*)(As + swz(wr * WTM + r * 32 + l32, koff));
 #pragma unroll
-          for (int c = 0; c < TN; ++c) bv[c] = *(const v8h*)(Bs + (wc * WTN + c * 32 + l32) * PITCH + koff);
+          for (int c = 0; c < TN; ++c) bv[c] = *(const v8h*)(Bs + swz(wc * WTN + c * 32 + l32, koff));
 #pragma unroll
           for (int r = 0; r < TM; ++r)
 #pragma unroll
@@ -163,30 +172,63 @@ struct SplitTile {
           v8bf ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
           for (int r = 0; r < TM; ++r) {
-            const char* p = As + (wr * WTM + r * 32 + l32) * PITCH + koff;
+            const char* p = As + swz(wr * WTM + r * 32 + l32, koff);
             ah[r] = *(const v8bf*)p;
             al[r] = *(const v8bf*)(p + PLANE_A);
           }
 #pragma unroll
           for (int c = 0; c < TN; ++c) {
-            const char* p = Bs + (wc * WTN + c * 32 + l32) * PITCH + koff;
+            const char* p = Bs + swz(wc * WTN + c * 32 + l32, koff);
             bh[c] = *(const v8bf*)p;
             bl[c] = *(const v8bf*)(p + PLANE_B);
           }
+          // the three products outermost: consecutive MFMAs on one accumulator
+          // are TM*TN issues apart instead of back to back
 #pragma unroll
           for (int r = 0; r < TM; ++r)
 #pragma unroll
-            for (int c = 0; c < TN; ++c) {
+            for (int c = 0; c < TN; ++c)
               acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < TM; ++r)
+#pragma unroll
+            for (int c = 0; c < TN; ++c)
               acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < TM; ++r)
+#pragma unroll
+            for (int c = 0; c < TN; ++c)
               acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
-            }
         }
+        mid(s);
       }
-      if (more) put(lds + (cur ^ 1) * STAGE);
+    };
+    // one k-tile: `nxt` holds tile kt+1 (to quantise now), `ldr` receives kt+2
+    auto step = [&](int kt, typename LA::Regs& nxa, typename LB::Regs& nxb, typename LA::Regs& lda_,
+                    typename LB::Regs& ldb_) {
+      const int cur = kt & 1;
+      if (kt + 2 < nk) load(lda_, ldb_, kt + 2);
+      char* other = lds + (cur ^ 1) * STAGE;
+      const bool more = kt + 1 < nk;
+      compute(lds + cur * STAGE, [&](int s) {
+        if (!more) return;
+        if (s == 0) store<LA>(nxa, other, PLANE_A, sA);
+        if (s == BK / 16 - 1) store<LB>(nxb, other + PLANES * PLANE_A, PLANE_B, sB);
+      });
       __syncthreads();
+    };
+    if (nk > 0) {
+      load(la0, lb0, 0);
+      store<LA>(la0, lds, PLANE_A, sA);
+      store<LB>(lb0, lds + PLANES * PLANE_A, PLANE_B, sB);
     }
-    const float alpha = a.alpha * post;
+    if (nk > 1) load(la1, lb1, 1);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt += 2) {
+      step(kt, la1, lb1, la0, lb0);
+      if (kt + 1 < nk) step(kt + 1, la0, lb0, la1, lb1);
+    }
+    const OutTile o = vbg_out(a, d, it, post);
 #pragma unroll
     for (int r = 0; r < TM; ++r)
 #pragma unroll
@@ -196,13 +238,14 @@ struct SplitTile {
           const int row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
           const int col = n0 + wc * WTN + c * 32 + l32;
           v4f v = {acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
-          vbg_store4(C, d.ldc, row, col, d.M, d.N, v, alpha, a.beta);
+          vbg_store4(o.p, o.ld, row, col, d.M, d.N, v, o.alpha, o.beta);
         }
   }
 };
 
 template <class Tile>
-__global__ void __launch_bounds__(VBG_THREADS) vbg_split_kernel(vbg_args a) {
+__global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+vbg_split_kernel(vbg_args a) {
   __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
   vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
 }

@@ -16,22 +16,72 @@ typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
 #define VBG_THREADS 256
 
 // ---------------------------------------------------------------------------
+// Wave-uniform views.  Work items, descriptors and operand pointers are the
+// same for every lane of a workgroup, but the compiler cannot prove it for
+// values loaded from global memory (the kernel also stores to global memory);
+// readfirstlane moves them to scalar registers, so buffer resources, k offsets
+// and loop bounds stay scalar and no per-lane address arithmetic is generated.
+__device__ __forceinline__ int vbg_u(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+template <class T>
+__device__ __forceinline__ T* vbg_uptr(T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T*)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int64_t vbg_u64(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ vbg_mat_desc vbg_udesc(const vbg_mat_desc& d) {
+  return {vbg_u(d.M), vbg_u(d.N), vbg_u(d.K), vbg_u(d.lda), vbg_u(d.ldb), vbg_u(d.ldc),
+          vbg_u(d.tiles_n), vbg_u(d.ksplit)};
+}
+
+__device__ __forceinline__ vbg_item vbg_uitem(const vbg_item& it) {
+  return {vbg_u(it.mat), vbg_u(it.tm), vbg_u(it.tn), vbg_u(it.kslice)};
+}
+
+// ---------------------------------------------------------------------------
+// Buffer resource over one operand: raw buffer loads take a per-lane 32-bit
+// byte offset (VGPR) plus a wave-uniform byte offset (SGPR), and return 0 for
+// any access past `bytes` instead of faulting.  Third dword 0x00020000 is the
+// gfx9-family data-format word for untyped dword access.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vbg_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// ---------------------------------------------------------------------------
 // Operand tile loader.  Stages a ROWS x BK slice of one operand -- rows are the
 // operand's M (for A) or N (for B) index "mn", columns the reduction index k --
-// through registers into LDS laid out [mn][k] with pitch P (k contiguous), so
-// that an MFMA lane reads 4 consecutive k with one ds_read_b128.
+// through registers.  r[v] holds 4 consecutive k of one row (coords(v) names
+// the row and the k-quad), which the tile writes to LDS laid out [mn][k] (k
+// contiguous), so that an MFMA lane reads consecutive k with one ds_read.
 //
-// KC = true : element (mn, k) at p[mn*ld + k]  (k contiguous in HBM: A^T, B)
-// KC = false: element (mn, k) at p[mn + k*ld]  (mn contiguous in HBM: A, B^T)
+// KC = true : element (mn, k) at p[mn*ld + k]  (k contiguous in HBM: A^T, B);
+//             one 16-byte buffer load per r[v].
+// KC = false: element (mn, k) at p[mn + k*ld]  (mn contiguous in HBM: A, B^T);
+//             4 dword loads per r[v], one per k, each wave-coalesced along mn.
 //
-// Rows past MN and k past K read as zero.  Every address stays inside the
-// operand: out-of-range rows are clamped to MN-1 and k to K-1 before the load
-// and the loaded value is then replaced by zero.
+// Addressing: init() fixes each lane's byte offset for its row and k-quad once
+// per work item (rows past MN clamped to MN-1: they only feed outputs that are
+// never stored); a k-tile's position is a wave-uniform scalar offset, so the
+// main loop issues no vector address arithmetic.  In the last (partial) k-tile
+// every k >= K reads as zero (out-of-range loads return 0 from the buffer
+// unit; the select guards data that lies inside the allocation).  Operands
+// must span less than 2^31 bytes (the plan checks).
 template <int ROWS, int BK, int P, bool KC>
 struct TileLoader {
   static constexpr int NV = ROWS * BK / (4 * VBG_THREADS);  // float4 groups per thread
   static_assert(NV >= 1 && ROWS * BK % (4 * VBG_THREADS) == 0, "tile too small for the block");
-  v4f r[NV];
+  typedef v4f Regs[NV];
+  __amdgpu_buffer_rsrc_t rsrc;
+  int voff[NV];
+  int ld4;
 
   __device__ __forceinline__ static void coords(int v, int& row, int& kq) {
     const int idx = threadIdx.x + VBG_THREADS * v;
@@ -44,48 +94,57 @@ struct TileLoader {
     }
   }
 
-  // ktail: this k-tile may extend past K (only the last one can).
-  __device__ __forceinline__ void load(const float* __restrict__ p, int ld, int mn0, int k0, int MN,
-                                       int K, bool ktail) {
+  // p: operand base; MN rows; K the operand's full reduction length.
+  __device__ __forceinline__ void init(const float* p, int ld, int mn0, int MN, int K) {
+    const uint32_t extent = (uint32_t)ld * (uint32_t)(KC ? MN : K) * 4u;
+    rsrc = vbg_rsrc(p, extent);
+    ld4 = ld * 4;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       int row, kq;
       coords(v, row, kq);
-      const int mn = mn0 + row;
-      const bool ok = mn < MN;
-      const int mnc = ok ? mn : MN - 1;
-      const int k = k0 + 4 * kq;
-      if (KC) {
-        const float* q = p + (size_t)mnc * ld;
-        if (!ktail) {
-          r[v] = *(const f4u*)(q + k);
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int kk = k + e;
-            const float x = q[kk < K ? kk : K - 1];
-            r[v][e] = kk < K ? x : 0.0f;
-          }
-        }
-      } else {
-        const float* q = p + mnc;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          int kk = k + e;
-          bool okk = true;
-          if (ktail) {
-            okk = kk < K;
-            kk = okk ? kk : K - 1;
-          }
-          const float x = q[(size_t)kk * ld];
-          r[v][e] = okk ? x : 0.0f;
-        }
-      }
-      if (!ok) r[v] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+      const int mnc = min(mn0 + row, MN - 1);
+      voff[v] = KC ? (mnc * ld + 4 * kq) * 4 : (mnc + 4 * kq * ld) * 4;
     }
   }
 
-  __device__ __forceinline__ void store(float* lds) const {
+  // A whole k-tile starting at k0 (k0 + BK <= K).
+  __device__ __forceinline__ void load(Regs& r, int k0) const {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if (KC) {
+        r[v] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff[v], k0 * 4, 0));
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          r[v][e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff[v], (k0 + e) * ld4, 0));
+      }
+    }
+  }
+
+  // The last k-tile [k0, K): k >= K reads as zero.
+  __device__ __forceinline__ void load_tail(Regs& r, int k0, int K) const {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      int row, kq;
+      coords(v, row, kq);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const bool ok = k0 + 4 * kq + e < K;
+        const unsigned bits = KC ? __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff[v] + 4 * e, k0 * 4, 0)
+                                 : __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff[v], (k0 + e) * ld4, 0);
+        const float x = __builtin_bit_cast(float, bits);
+        r[v][e] = ok ? x : 0.0f;
+      }
+    }
+  }
+
+  __device__ __forceinline__ void fetch(Regs& r, int k0, int K) const {
+    if (k0 + BK <= K) load(r, k0);
+    else load_tail(r, k0, K);
+  }
+
+  __device__ __forceinline__ static void store(const Regs& r, float* lds) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       int row, kq;
@@ -126,10 +185,12 @@ __device__ __forceinline__ void vbg_persistent_loop(const vbg_args& a, Body&& bo
       s_item = it;
     }
     __syncthreads();
-    const int it = s_item;
+    // the item index is wave-uniform; saying so lets the compiler keep the
+    // item, its descriptor, pointers and k offsets in scalar registers
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
     if (it < 0) break;
-    body(a.items[it]);
+    body(vbg_uitem(a.items[it]));
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -154,6 +215,23 @@ __device__ __forceinline__ KRange vbg_k_range(int K, int ksplit, int slice) {
   const int per = (ktiles + ksplit - 1) / ksplit;
   const int kb = min(K, slice * per * BK);
   return {kb, min(K, kb + per * BK)};
+}
+
+// Where an item's tile goes: the matrix's C with alpha*scale and beta, or --
+// for one K slice of a split matrix -- its FP32 partial panel in the workspace
+// (column-major, ld = M; slice s at offset s*M*N), scaled by `scale` only; the
+// split-K reduction applies alpha and beta.
+struct OutTile {
+  float* p;
+  int ld;
+  float alpha, beta;
+  bool partial;
+};
+__device__ __forceinline__ OutTile vbg_out(const vbg_args& a, const vbg_mat_desc& d,
+                                           const vbg_item& it, float scale) {
+  if (d.ksplit > 1)
+    return {a.ws + vbg_u64(a.ws_off[it.mat]) + (size_t)it.kslice * d.M * d.N, d.M, scale, 0.0f, true};
+  return {vbg_uptr(a.C[it.mat]), d.ldc, a.alpha * scale, a.beta, false};
 }
 
 // Epilogue for a 4-row column segment: C[row..row+3, col] = alpha*v + beta*C.

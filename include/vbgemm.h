@@ -86,13 +86,18 @@ typedef struct vbg_opts {
   int fixed_tile;       /* FIXED: vbg_tile id */
   unsigned tile_mask;   /* AUTO: bit t enables tile t (0 = all) */
   int big_tile_min_mn;  /* AUTO: min(M,N) at which 128x128 becomes eligible (0 = default) */
-  int split_k_min;      /* AUTO: split K for matrices with K >= this and few tiles (0 = default,
-                           <0 = never split) */
+  int split_k_min;      /* split K of matrices with K >= this whose one item would be the
+                           launch's critical path (0 = 256, <0 = never split) */
   int grid_wgs;         /* persistent grid size per kernel (0 = CUs x occupancy) */
   int xcd_queues;       /* 1 = XCD-aware per-XCD work queues with stealing, 0 = one queue */
   int host_only;        /* 1 = plan on the host only (no device upload, cannot run);
                            for inspection and tests on machines without a GPU */
   int num_cus;          /* host_only: CU count to plan for (0 = 256) */
+  int force_ksplit;     /* >0: split every matrix's K into this many slices (capped at its
+                           k-tile count; tests and tuning); 0 = the planner decides */
+  int tile_group;       /* tile order inside a matrix: runs of this many tile rows, column by
+                           column (0 = 8; 1 = plain row-major), so the workgroups of one XCD
+                           that run consecutive items share A and B panels in its L2 */
 } vbg_opts;
 
 typedef struct vbg_plan vbg_plan;

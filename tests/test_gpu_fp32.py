@@ -122,3 +122,16 @@ def test_fp32_zero_k_and_empty(bg, orc, gpu):
     p = bg.Plan([], [], [])
     e = torch.zeros(1, dtype=torch.int64, device="cuda")
     p.run(e, e, e, 1.0, 0.0)
+
+
+@pytest.mark.parametrize("tile", ["16x16", "32x32", "64x64", "128x128"])
+def test_fp32_split_k_forced(bg, orc, gpu, tile):
+    """K cut into slices whose FP32 partial panels are summed by the split-K
+    reduction (alpha and beta applied there)."""
+    say(f"[fp32] split-K tile={tile}")
+    M, N, K = EDGE_SHAPES
+    K = [k * 9 + 1 for k in K]
+    vb, C0, plan = run_case(bg, orc, M, N, K, "N", "N", 1.5, 0.5, 8, policy="fixed",
+                            fixed_tile=tile, force_ksplit=3)
+    assert (plan.ksplit() > 1).sum() >= 8
+    oracle_check(orc, vb, 1.5, 0.5, C0)

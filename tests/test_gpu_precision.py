@@ -139,3 +139,22 @@ def test_quantisation_elementwise_via_identity(bg, orc, gpu, prec):
             s = orc.fp8_scale(np.abs(A).max())
             want = orc.fp8_scaled(A, s).astype(np.float64) / np.float64(s)
         np.testing.assert_allclose(got, want, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16", "fp8"])
+def test_split_precision_split_k(bg, orc, gpu, prec):
+    import torch
+    from batchgemm_amd.workloads import VBatch
+    say(f"[{prec}] split-K")
+    M, N, K = [64, 100, 130], [64, 33, 128], [700, 333, 1025]
+    vb = VBatch(M, N, K).fill("normal", seed=21, c_dist="uniform")
+    C0 = vb.C.cpu().numpy()
+    plan = bg.Plan(M, N, K, prec=prec, force_ksplit=4)
+    assert (plan.ksplit() == 4).all()
+    plan.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
+    torch.cuda.synchronize()
+    for b in range(3):
+        A = op_matrix(vb, "A", b); B = op_matrix(vb, "B", b)
+        c0 = C0[int(vb.offC[b]): int(vb.offC[b]) + M[b] * N[b]].reshape(N[b], M[b]).T.astype(np.float64)
+        got = vb.host_flat("C", b).reshape(N[b], M[b]).T
+        assert orc.rel_fro(got, 1.5 * emulate(orc, prec, A, B) + 0.5 * c0) <= EMUL_BOUND[prec]
