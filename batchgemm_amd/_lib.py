@@ -13,7 +13,8 @@ import os
 import torch  # noqa: F401  (must precede the CDLL load, see module docstring)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libvbgemm.so")
+# VBG_LIBRARY selects another build of the same ABI (A/B comparisons of kernels).
+LIB_PATH = os.environ.get("VBG_LIBRARY") or os.path.join(_HERE, "libvbgemm.so")
 
 VBG_OK = 0
 VBG_ERR_INVALID_ARG = -1

@@ -16,7 +16,6 @@ typedef unsigned u32;
 typedef u32 v4u __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
 
-__device__ __forceinline__ u16 ld16(const u16* p) { return *p; }
 
 // LDS row layout shared by loader and tile: rows of BK BF16 (BK*2 bytes) in
 // 16-byte units; unit u of row r sits at unit u ^ ((r / 2) % 8): rows are 128
@@ -29,17 +28,22 @@ __device__ __forceinline__ int bswz(int row, int byte) {
 }
 
 // BF16 operand loader.  LDS plane layout [mn][k], k contiguous, PITCH bytes.
-// KC (k contiguous in HBM): a thread loads 8 consecutive k of one row.
-// MC (mn contiguous in HBM): a thread loads 8 consecutive mn at k and at k+1,
+// KC (k contiguous in HBM): a lane loads 8 consecutive k of one row.
+// MC (mn contiguous in HBM): a lane loads 8 consecutive mn at k and at k+1,
 // then interleaves them into 8 (k, k+1) pairs, one 4-byte LDS write per row.
-// Rows past MN are loaded from clamped addresses (they only feed outputs that
-// are never stored); k past K is zero.
+// Buffer loads: a lane's byte offset is fixed once per work item and the
+// k-tile position is a wave-uniform scalar offset.  Rows past MN are clamped
+// (they only feed outputs that are never stored); k past K is zero; a vector
+// that straddles MN or K falls back to 2-byte loads.
 template <int ROWS, int BK, int PITCH, bool KC>
 struct Bf16Loader {
   static constexpr int UNITS = KC ? ROWS * BK / 8 : ROWS * BK / 16;
   static constexpr int NV = UNITS / VBG_THREADS;
   static_assert(NV >= 1 && UNITS % VBG_THREADS == 0, "tile too small for the block");
-  v4u r[KC ? NV : 2 * NV];
+  typedef v4u Regs[KC ? NV : 2 * NV];
+  __amdgpu_buffer_rsrc_t rsrc;
+  int voff[NV];
+  int ld2, MN, mn0;
 
   __device__ __forceinline__ static void coords(int v, int& row, int& k) {
     const int idx = threadIdx.x + VBG_THREADS * v;
@@ -52,25 +56,46 @@ struct Bf16Loader {
     }
   }
 
-  __device__ __forceinline__ void load(const u16* __restrict__ p, int ld, int mn0, int k0, int MN,
-                                       int K, bool ktail) {
+  __device__ __forceinline__ void init(const u16* p, int ld, int mn0_, int MN_, int K) {
+    rsrc = vbg_rsrc(p, (uint32_t)ld * (uint32_t)(KC ? MN_ : K) * 2u);
+    ld2 = ld * 2;
+    MN = MN_;
+    mn0 = mn0_;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      int row, kk;
+      coords(v, row, kk);
+      if (KC) {
+        const int mnc = min(mn0 + row, MN - 1);
+        voff[v] = (mnc * ld + kk) * 2;
+      } else {
+        voff[v] = (min(mn0 + row, MN - 1) + kk * ld) * 2;
+      }
+    }
+  }
+
+  __device__ __forceinline__ u32 ld16(int off, int soff) const {
+    return __builtin_amdgcn_raw_buffer_load_b16(rsrc, off, soff, 0);
+  }
+  __device__ __forceinline__ v4u ld128(int off, int soff) const {
+    return __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, soff, 0));
+  }
+
+  __device__ __forceinline__ void fetch(Regs& r, int k0, int K) const {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       int row, kk;
       coords(v, row, kk);
       const int k = k0 + kk;
       if (KC) {
-        const int mn = min(mn0 + row, MN - 1);
-        const u16* q = p + (size_t)mn * ld;
-        if (!ktail || k + 7 < K) {
-          r[v] = *(const v4u*)(q + k);
+        if (k + 8 <= K) {
+          r[v] = ld128(voff[v], k0 * 2);
         } else {
           u32 w[4];
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            const int ka = k + 2 * e, kb = ka + 1;
-            const u32 lo = ka < K ? ld16(q + ka) : 0u;
-            const u32 hi = kb < K ? ld16(q + kb) : 0u;
+            const u32 lo = k + 2 * e < K ? ld16(voff[v] + 4 * e, k0 * 2) : 0u;
+            const u32 hi = k + 2 * e + 1 < K ? ld16(voff[v] + 4 * e + 2, k0 * 2) : 0u;
             w[e] = lo | (hi << 16);
           }
           r[v] = v4u{w[0], w[1], w[2], w[3]};
@@ -79,30 +104,28 @@ struct Bf16Loader {
         const int mn = mn0 + row;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          const int kj = k + j;
-          const bool okk = !ktail || kj < K;
-          const u16* q = p + (size_t)(okk ? kj : K - 1) * ld;
+          const int soff = (k0 + j) * ld2;
           v4u x;
           if (mn + 7 < MN) {
-            x = *(const v4u*)(q + mn);
-          } else {
+            x = ld128(voff[v], soff);
+          } else {  // straddles MN: element by element, clamped
             u32 w[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-              const u32 lo = ld16(q + min(mn + 2 * e, MN - 1));
-              const u32 hi = ld16(q + min(mn + 2 * e + 1, MN - 1));
-              w[e] = lo | (hi << 16);
+              const int a = min(mn + 2 * e, MN - 1) - min(mn, MN - 1);
+              const int b = min(mn + 2 * e + 1, MN - 1) - min(mn, MN - 1);
+              w[e] = ld16(voff[v] + 2 * a, soff) | (ld16(voff[v] + 2 * b, soff) << 16);
             }
             x = v4u{w[0], w[1], w[2], w[3]};
           }
-          if (!okk) x = v4u{0u, 0u, 0u, 0u};
+          if (k + j >= K) x = v4u{0u, 0u, 0u, 0u};
           r[2 * v + j] = x;
         }
       }
     }
   }
 
-  __device__ __forceinline__ void store(char* plane) const {
+  __device__ __forceinline__ static void store(const Regs& r, char* plane) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       int row, kk;
@@ -194,12 +217,15 @@ struct Bf16Tile {
 
     LA la;
     LB lb;
+    la.init(A, d.lda, m0, d.M, d.K);
+    lb.init(B, d.ldb, n0, d.N, d.K);
+    typename LA::Regs ra;
+    typename LB::Regs rb;
     if (nk > 0) {
-      const bool tail0 = kb + BK > ke;
-      la.load(A, d.lda, m0, kb, d.M, ke, tail0);
-      lb.load(B, d.ldb, n0, kb, d.N, ke, tail0);
-      la.store(lds);
-      lb.store(lds + PLANE_A);
+      la.fetch(ra, kb, ke);
+      lb.fetch(rb, kb, ke);
+      LA::store(ra, lds);
+      LB::store(rb, lds + PLANE_A);
     }
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
@@ -207,9 +233,8 @@ struct Bf16Tile {
       const bool more = kt + 1 < nk;
       if (more) {
         const int k0 = kb + (kt + 1) * BK;
-        const bool tail = k0 + BK > ke;
-        la.load(A, d.lda, m0, k0, d.M, ke, tail);
-        lb.load(B, d.ldb, n0, k0, d.N, ke, tail);
+        la.fetch(ra, k0, ke);
+        lb.fetch(rb, k0, ke);
       }
       const char* As = lds + cur * STAGE;
       const char* Bs = As + PLANE_A;
@@ -229,8 +254,8 @@ struct Bf16Tile {
       }
       if (more) {
         char* nxt = lds + (cur ^ 1) * STAGE;
-        la.store(nxt);
-        lb.store(nxt + PLANE_A);
+        LA::store(ra, nxt);
+        LB::store(rb, nxt + PLANE_A);
       }
       __syncthreads();
     }

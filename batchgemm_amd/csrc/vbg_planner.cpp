@@ -420,7 +420,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
                    [](const Launch& x, const Launch& y) { return x.cost > y.cost; });
   for (int b = 0; b < batch; ++b) {
     if (hp.desc[b].ksplit < 2) continue;
-    const int64_t n = (int64_t)M[b] * N[b], target = 1 << 16;
+    const int64_t n = (int64_t)M[b] * N[b], target = 1 << 12;  // 16 elements per thread
     for (int64_t s0 = 0; s0 < n; s0 += target) hp.red_chunks.push_back({b, 0, s0, std::min(n, s0 + target)});
   }
   if (prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3) {

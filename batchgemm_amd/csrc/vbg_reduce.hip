@@ -7,6 +7,22 @@
 
 namespace {
 
+__device__ __forceinline__ void reduce_store(float* const* C, int mat, size_t ci, float s,
+                                             float alpha, float beta, int out_bf16) {
+  if (out_bf16) {
+    unsigned short* p = (unsigned short*)C[mat] + ci;
+    float o = alpha * s;
+    if (beta != 0.0f) o += beta * __uint_as_float((unsigned)*p << 16);
+    const __bf16 h = (__bf16)o;
+    *p = *(const unsigned short*)&h;
+  } else {
+    float* p = C[mat] + ci;
+    float o = alpha * s;
+    if (beta != 0.0f) o += beta * *p;
+    *p = o;
+  }
+}
+
 __global__ void __launch_bounds__(VBG_THREADS)
     vbg_splitk_reduce(const vbg_chunk* ch, const vbg_mat_desc* desc, const float* ws,
                       const int64_t* ws_off, float* const* C, float alpha, float beta, int out_bf16) {
@@ -14,22 +30,18 @@ __global__ void __launch_bounds__(VBG_THREADS)
   const vbg_mat_desc d = desc[c.mat];
   const float* W = ws + ws_off[c.mat];
   const size_t mn = (size_t)d.M * d.N;
-  for (int64_t e = c.begin + threadIdx.x; e < c.end; e += VBG_THREADS) {
-    const int i = (int)(e % d.M), j = (int)(e / d.M);
-    float s = 0.0f;
-    for (int k = 0; k < d.ksplit; ++k) s += W[k * mn + e];
-    const size_t ci = (size_t)j * d.ldc + i;
-    if (out_bf16) {
-      unsigned short* p = (unsigned short*)C[c.mat] + ci;
-      float o = alpha * s;
-      if (beta != 0.0f) o += beta * __uint_as_float((unsigned)*p << 16);
-      const __bf16 h = (__bf16)o;
-      *p = *(const unsigned short*)&h;
-    } else {
-      float* p = C[c.mat] + ci;
-      float o = alpha * s;
-      if (beta != 0.0f) o += beta * *p;
-      *p = o;
+  if (d.ldc == d.M) {  // packed C: the panel index is the C index
+    for (int64_t e = c.begin + threadIdx.x; e < c.end; e += VBG_THREADS) {
+      float s = 0.0f;
+      for (int k = 0; k < d.ksplit; ++k) s += W[k * mn + e];
+      reduce_store(C, c.mat, (size_t)e, s, alpha, beta, out_bf16);
+    }
+  } else {
+    for (int64_t e = c.begin + threadIdx.x; e < c.end; e += VBG_THREADS) {
+      const int i = (int)(e % d.M), j = (int)(e / d.M);
+      float s = 0.0f;
+      for (int k = 0; k < d.ksplit; ++k) s += W[k * mn + e];
+      reduce_store(C, c.mat, (size_t)j * d.ldc + i, s, alpha, beta, out_bf16);
     }
   }
 }

@@ -13,6 +13,7 @@
 #include "vbg_tiles.h"
 #include "../../include/vbgemm.h"
 
+
 namespace {
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
@@ -34,17 +35,24 @@ enum { MODE_BF16X3 = 0, MODE_FP16 = 1, MODE_FP8 = 2 };
 
 template <int MODE>
 struct Mode;
+// STAGES: register stages of the pipeline.  2: tile kt+1 is quantised between
+// the MFMA steps of tile kt while tile kt+2 loads; 1: tile kt+1 loads during
+// tile kt and is quantised after it.  Measured on the unbalanced B=2048 batch:
+// bf16x3 is faster with 1 (its 64 KB of LDS holds occupancy at 2 either way,
+// and the second stage costs registers), FP16/FP8 with 2.  WAVES: the
+// register-allocation target handed to the compiler (bf16x3's 3 keeps its
+// accumulators in AGPRs, the allocation that was measured fastest).
 template <>
 struct Mode<MODE_BF16X3> {
-  static constexpr int EB = 2, PLANES = 2;
+  static constexpr int EB = 2, PLANES = 2, STAGES = 1, WAVES = 3;
 };
 template <>
 struct Mode<MODE_FP16> {
-  static constexpr int EB = 2, PLANES = 1;
+  static constexpr int EB = 2, PLANES = 1, STAGES = 2, WAVES = 2;
 };
 template <>
 struct Mode<MODE_FP8> {
-  static constexpr int EB = 1, PLANES = 1;
+  static constexpr int EB = 1, PLANES = 1, STAGES = 2, WAVES = 2;
 };
 
 // LDS layout per operand plane: [mn][k] with k contiguous, rows of BK*EB bytes
@@ -56,6 +64,7 @@ struct Mode<MODE_FP8> {
 template <int BM, int BN, int MODE, bool TA, bool TB>
 struct SplitTile {
   static constexpr int BK = 32;
+  static constexpr int MODE_ID = MODE;
   static constexpr int EB = Mode<MODE>::EB, PLANES = Mode<MODE>::PLANES;
   static constexpr int PITCH = BK * EB;  // bytes per LDS row (no padding: swizzled)
   static constexpr int UNIT = PITCH / 4;  // bytes per MFMA operand read
@@ -217,6 +226,7 @@ struct SplitTile {
       });
       __syncthreads();
     };
+    if constexpr (Mode<MODE>::STAGES >= 2) {
     if (nk > 0) {
       load(la0, lb0, 0);
       store<LA>(la0, lds, PLANE_A, sA);
@@ -227,6 +237,31 @@ struct SplitTile {
     for (int kt = 0; kt < nk; kt += 2) {
       step(kt, la1, lb1, la0, lb0);
       if (kt + 1 < nk) step(kt + 1, la0, lb0, la1, lb1);
+    }
+    } else {
+    // one register stage: tile kt+1 is loaded while tile kt computes and is
+    // quantised into the other LDS buffer after the MFMAs
+    (void)la1;
+    (void)lb1;
+    (void)step;
+    if (nk > 0) {
+      load(la0, lb0, 0);
+      store<LA>(la0, lds, PLANE_A, sA);
+      store<LB>(lb0, lds + PLANES * PLANE_A, PLANE_B, sB);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) load(la0, lb0, kt + 1);
+      compute(lds + cur * STAGE, [](int) {});
+      if (more) {
+        char* other = lds + (cur ^ 1) * STAGE;
+        store<LA>(la0, other, PLANE_A, sA);
+        store<LB>(lb0, other + PLANES * PLANE_A, PLANE_B, sB);
+      }
+      __syncthreads();
+    }
     }
     const OutTile o = vbg_out(a, d, it, post);
 #pragma unroll
@@ -244,7 +279,7 @@ struct SplitTile {
 };
 
 template <class Tile>
-__global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(Mode<Tile::MODE_ID>::WAVES)))
 vbg_split_kernel(vbg_args a) {
   __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
   vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });

@@ -83,11 +83,19 @@ struct TileLoader {
   int voff[NV];
   int ld4;
 
+  // MC with ROWS >= 32: each half-wave takes 32 consecutive rows (128-byte
+  // coalesced loads) and the two halves of a wave take neighbouring k-quads,
+  // so one LDS store instruction covers every bank.
   __device__ __forceinline__ static void coords(int v, int& row, int& kq) {
     const int idx = threadIdx.x + VBG_THREADS * v;
     if (KC) {
       row = idx / (BK / 4);
       kq = idx % (BK / 4);
+    } else if (ROWS >= 32) {
+      constexpr int RB = ROWS / 32;
+      const int l = idx & 63, w = idx >> 6;
+      row = (l & 31) + 32 * (w % RB);
+      kq = 2 * (w / RB) + (l >> 5);
     } else {
       row = idx % ROWS;
       kq = idx / ROWS;
