@@ -59,6 +59,7 @@ class vbg_opts(ctypes.Structure):
         ("num_cus", ctypes.c_int),
         ("force_ksplit", ctypes.c_int),
         ("tile_group", ctypes.c_int),
+        ("edge_tiles", ctypes.c_int),
     ]
 
 

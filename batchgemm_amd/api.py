@@ -118,6 +118,14 @@ class Plan:
                                    len(out)), "vbg_plan_query")
         return out[:self.batch]
 
+    def tiled_items(self) -> np.ndarray:
+        """Work items as rows (mat, tile id, tile_m, tile_n, k slice)."""
+        n = self.n_items
+        out = np.zeros(max(5 * n, 1), np.int32)
+        check(lib().vbg_plan_query(self._p, 6, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   len(out)), "vbg_plan_query")
+        return out[:5 * n].reshape(n, 5)
+
     def items(self) -> np.ndarray:
         n = self.n_items
         out = np.zeros(max(5 * n, 1), np.int32)

@@ -309,6 +309,19 @@ int vbg_plan_query(const vbg_plan* p, int what, int* out, int out_len) {
       for (int b = 0; b < hp.batch; ++b) out[b] = what == 2 ? hp.tile_of[b] : hp.desc[b].ksplit;
       return hp.batch;
     }
+    case 6: {
+      if (!out || out_len < 5 * n) return fail(VBG_ERR_INVALID_ARG, "out too small");
+      int w = 0;
+      for (const auto& L : hp.launches)
+        for (const auto& it : L.items) {
+          out[w++] = it.mat;
+          out[w++] = L.tile;
+          out[w++] = it.tm;
+          out[w++] = it.tn;
+          out[w++] = it.kslice;
+        }
+      return w;
+    }
     case 4: {
       if (!out || out_len < 5 * n) return fail(VBG_ERR_INVALID_ARG, "out too small");
       int w = 0;

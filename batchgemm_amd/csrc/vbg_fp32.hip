@@ -109,17 +109,12 @@ struct Fp32MfmaTile {
     // 32x32x2 f32 accumulator layout: register 4g+e of lane l holds
     // row 8g + 4*(l/32) + e, column l%32 of the 32x32 block.
     const OutTile o = vbg_out(a, d, it, 1.0f);
-#pragma unroll
-    for (int r = 0; r < TM; ++r)
-#pragma unroll
-      for (int s = 0; s < TN; ++s)
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
-          const int col = n0 + wc * WTN + s * 32 + l32;
-          v4f v = {acc[r][s][4 * g], acc[r][s][4 * g + 1], acc[r][s][4 * g + 2], acc[r][s][4 * g + 3]};
-          vbg_store4(o.p, o.ld, row, col, d.M, d.N, v, o.alpha, o.beta);
-        }
+    vbg_epilogue<TM * TN * 4>(o, d.M, d.N, [&](int sg, int& row, int& col, v4f& v) {
+      const int r = sg / (TN * 4), s = (sg / 4) % TN, g = sg % 4;
+      row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
+      col = n0 + wc * WTN + s * 32 + l32;
+      v = v4f{acc[r][s][4 * g], acc[r][s][4 * g + 1], acc[r][s][4 * g + 2], acc[r][s][4 * g + 3]};
+    });
   }
 };
 

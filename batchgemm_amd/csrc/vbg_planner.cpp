@@ -367,46 +367,76 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
 
   // 2. group matrices by kernel family/tile; 3. LPT over XCD queues.
   hp.nq = o.xcd_queues ? VBG_NUM_XCD : 1;
+  // 2a. parts: the items of one matrix that run on one tile shape.  With
+  // edge tiles on, a matrix on a tile with a 128-wide side whose last tile row
+  // (or column) holds at most 64 rows (columns) covers that strip with
+  // 64-wide tiles instead, in the launch of that shape: about half the padded
+  // MFMA work of a random-size edge goes away.  Item coordinates are in units
+  // of the item's own tile (128*q = 64*(2q)).
+  struct Part {
+    int b;
+    int tm0, tm1, tn0, tn1;  // tile ranges in the part's own tile units
+  };
+  std::vector<std::vector<Part>> parts(NTILES);
+  // default (0): on for the FP32 and BF16 families; the split family measured
+  // faster without (its extra small-tile launches cost more than the padding)
+  const bool edges = fam != FAM_SEQ && (o.edge_tiles > 0 || (o.edge_tiles == 0 && fam != FAM_SPLIT));
+  for (int b = 0; b < batch; ++b) {
+    if (M[b] == 0 || N[b] == 0) continue;
+    const int t = tile[b];
+    hp.tile_of[b] = t;
+    hp.desc[b].tiles_n = cdiv(N[b], kTileBN[t]);
+    const int BMt = kTileBM[t], BNt = kTileBN[t];
+    const int r = M[b] % BMt, s = N[b] % BNt;
+    const int t_bot = BNt == 128 ? T64x128 : T64;  // 64 x BN
+    const int t_rgt = BMt == 128 ? T128x64 : T64;  // BM x 64
+    const bool half_m = edges && BMt == 128 && r > 0 && r <= 64 && tile_allowed(prec, t_bot, o.tile_mask) &&
+                        (BNt == 128 || tile_allowed(prec, T64, o.tile_mask));
+    const bool half_n = edges && BNt == 128 && s > 0 && s <= 64 && tile_allowed(prec, t_rgt, o.tile_mask) &&
+                        (BMt == 128 || tile_allowed(prec, T64, o.tile_mask));
+    const int rows = half_m ? M[b] / BMt : cdiv(M[b], BMt);
+    const int cols = half_n ? N[b] / BNt : cdiv(N[b], BNt);
+    if (rows > 0 && cols > 0) parts[t].push_back({b, 0, rows, 0, cols});
+    if (half_m && cols > 0) parts[t_bot].push_back({b, 2 * rows, 2 * rows + 1, 0, cols});
+    if (half_n && rows > 0) parts[t_rgt].push_back({b, 0, rows, 2 * cols, 2 * cols + 1});
+    if (half_m && half_n) parts[T64].push_back({b, 2 * rows, 2 * rows + 1, 2 * cols, 2 * cols + 1});
+  }
+
+  // 2b. one persistent launch per tile shape; 3. LPT over XCD queues.
   for (int t = 0; t < NTILES; ++t) {
-    std::vector<int> mats;
-    for (int b = 0; b < batch; ++b)
-      if (tile[b] == t && M[b] > 0 && N[b] > 0) mats.push_back(b);
-    if (mats.empty()) continue;
-    for (int b : mats) {
-      hp.tile_of[b] = t;
-      hp.desc[b].tiles_n = cdiv(N[b], kTileBN[t]);
-    }
+    std::vector<Part>& ps = parts[t];
+    if (ps.empty()) continue;
+    auto pcost = [&](const Part& x) {
+      return (double)(x.tm1 - x.tm0) * (x.tn1 - x.tn0) * hp.desc[x.b].ksplit * slice_cost(t, x.b);
+    };
     Launch L;
     L.tile = t;
     L.family = fam;
-    // longest matrices first, ties by index for determinism
-    std::stable_sort(mats.begin(), mats.end(), [&](int x, int y) {
-      return matrix_cost(t, sh[x]) > matrix_cost(t, sh[y]);
-    });
-    std::vector<std::vector<int>> qm(hp.nq);
+    // longest parts first, ties by matrix index for determinism
+    std::stable_sort(ps.begin(), ps.end(), [&](const Part& x, const Part& y) { return pcost(x) > pcost(y); });
+    std::vector<std::vector<Part>> qm(hp.nq);
     std::vector<double> load(hp.nq, 0.0);
-    for (int b : mats) {
+    for (const Part& x : ps) {
       const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      qm[q].push_back(b);
-      load[q] += matrix_cost(t, sh[b]);
-      L.cost += matrix_cost(t, sh[b]);
+      qm[q].push_back(x);
+      load[q] += pcost(x);
+      L.cost += pcost(x);
     }
     L.q_begin.push_back(0);
     for (int q = 0; q < hp.nq; ++q) {
-      // per-item cost decreasing; a matrix's tiles stay together
-      std::stable_sort(qm[q].begin(), qm[q].end(), [&](int x, int y) {
-        return slice_cost(t, x) > slice_cost(t, y);
+      // per-item cost decreasing; a part's tiles stay together
+      std::stable_sort(qm[q].begin(), qm[q].end(), [&](const Part& x, const Part& y) {
+        return slice_cost(t, x.b) > slice_cost(t, y.b);
       });
       // tiles in runs of `grp` tile rows, column by column: the ~64 items an
       // XCD runs at once form a grp x (64/grp) block of the matrix, touching
       // grp A panels and 64/grp B panels instead of 64/tn rows x tn columns
       const int grp = o.tile_group > 0 ? o.tile_group : 8;
-      for (int b : qm[q]) {
-        const int tm_n = cdiv(M[b], kTileBM[t]), tn_n = cdiv(N[b], kTileBN[t]);
-        for (int s = 0; s < hp.desc[b].ksplit; ++s)
-          for (int i0 = 0; i0 < tm_n; i0 += grp)
-            for (int j = 0; j < tn_n; ++j)
-              for (int i = i0; i < std::min(tm_n, i0 + grp); ++i) L.items.push_back({b, i, j, s});
+      for (const Part& x : qm[q]) {
+        for (int sl = 0; sl < hp.desc[x.b].ksplit; ++sl)
+          for (int i0 = x.tm0; i0 < x.tm1; i0 += grp)
+            for (int j = x.tn0; j < x.tn1; ++j)
+              for (int i = i0; i < std::min(x.tm1, i0 + grp); ++i) L.items.push_back({x.b, i, j, sl});
       }
       L.q_begin.push_back((int)L.items.size());
     }

@@ -264,17 +264,12 @@ struct SplitTile {
     }
     }
     const OutTile o = vbg_out(a, d, it, post);
-#pragma unroll
-    for (int r = 0; r < TM; ++r)
-#pragma unroll
-      for (int c = 0; c < TN; ++c)
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
-          const int col = n0 + wc * WTN + c * 32 + l32;
-          v4f v = {acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
-          vbg_store4(o.p, o.ld, row, col, d.M, d.N, v, o.alpha, o.beta);
-        }
+    vbg_epilogue<TM * TN * 4>(o, d.M, d.N, [&](int sg, int& row, int& col, v4f& v) {
+      const int r = sg / (TN * 4), c = (sg / 4) % TN, g = sg % 4;
+      row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
+      col = n0 + wc * WTN + c * 32 + l32;
+      v = v4f{acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
+    });
   }
 };
 

@@ -12,6 +12,7 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 // 4-byte-aligned float4 for HBM accesses at arbitrary element offsets (ld and
 // tile origins need not be multiples of 4); gfx950 runs these as dwordx4.
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef unsigned v4u32_t __attribute__((ext_vector_type(4)));
 
 #define VBG_THREADS 256
 
@@ -262,6 +263,67 @@ __device__ __forceinline__ void vbg_store4(float* C, int ldc, int row, int col, 
         float o = alpha * v[e];
         if (beta != 0.0f) o += beta * c[e];
         c[e] = o;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tile epilogue over NSEG 4-row column segments of the output (segment s:
+// rows row(s)..row(s)+3 of column col(s), values v(s)): C = alpha*v + beta*C.
+// Buffer accesses over the output's extent (ld * N floats); segments are
+// handled in chunks of 8 whose C loads (beta != 0) are all issued before any
+// store, so a tile costs two memory round trips instead of one per segment.
+// A segment crossing row M is written element by element.
+template <int NSEG, class Seg>
+__device__ __forceinline__ void vbg_epilogue(const OutTile& o, int M, int N, Seg&& seg) {
+  const __amdgpu_buffer_rsrc_t rs = vbg_rsrc(o.p, (uint32_t)o.ld * (uint32_t)N * 4u);
+  constexpr int CH = NSEG < 8 ? NSEG : 8;
+  static_assert(NSEG % CH == 0, "segments must split into whole chunks");
+#pragma unroll
+  for (int c0 = 0; c0 < NSEG; c0 += CH) {
+    v4f old[CH];
+    int off[CH];
+    bool full[CH], any[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      int row, col;
+      v4f v;
+      seg(c0 + j, row, col, v);
+      any[j] = col < N && row < M;
+      full[j] = any[j] && row + 3 < M;
+      off[j] = (col * o.ld + row) * 4;
+      old[j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+      if (o.beta != 0.0f && any[j]) {
+        if (full[j]) {
+          old[j] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rs, off[j], 0, 0));
+        } else {
+          const float* q = o.p + (size_t)col * o.ld + row;
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (row + e < M) old[j][e] = q[e];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      int row, col;
+      v4f v;
+      seg(c0 + j, row, col, v);
+      if (!any[j]) continue;
+      v4f r = v * o.alpha;
+      if (o.beta != 0.0f) r += o.beta * old[j];
+      if (full[j]) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32_t, r), rs, off[j], 0, 0);
+      } else {
+        // Edge segments go through plain global accesses: per-element dword
+        // buffer stores of a vector's lanes were compiled to store lane 0 to
+        // every offset (hipcc for gfx950), and edges are rare enough that the
+        // flat path costs nothing measurable.
+        float* q = o.p + (size_t)col * o.ld + row;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (row + e < M) q[e] = r[e];
       }
     }
   }

@@ -98,6 +98,9 @@ typedef struct vbg_opts {
   int tile_group;       /* tile order inside a matrix: runs of this many tile rows, column by
                            column (0 = 8; 1 = plain row-major), so the workgroups of one XCD
                            that run consecutive items share A and B panels in its L2 */
+  int edge_tiles;       /* cover a last tile row / column of <= 64 rows / columns with 64-wide
+                           tiles (less padded work): > 0 on, < 0 off, 0 = per-family default
+                           (on for FP32 and BF16 operands, off for bf16x3 / FP16 / FP8) */
 } vbg_opts;
 
 typedef struct vbg_plan vbg_plan;
@@ -151,6 +154,8 @@ int vbg_sgemm_vbatched_oneshot(int batch, const int* M, const int* N, const int*
  *   2 tile id per matrix (out[batch])  3 persistent grid size of launch 0
  *   4 work items as (mat, tile_m, tile_n, k_begin, k_end) x n (out[5*n])
  *   5 split-K parts per matrix (out[batch])
+ *   6 work items as (mat, tile id, tile_m, tile_n, k slice) x n (out[5*n]); tile_m / tile_n
+ *     count in units of that item's tile shape
  * Returns the number of ints written (or the scalar for 0, 1, 3). */
 int vbg_plan_query(const vbg_plan* plan, int what, int* out, int out_len);
 

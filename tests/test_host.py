@@ -34,21 +34,29 @@ def test_status_strings(bg):
         assert L.vbg_status_string(code) == name
 
 
-def _cover(plan, M, N):
-    """Every (matrix, tile) covered exactly once, no item outside its matrix."""
-    items = plan.items()
-    tiles = plan.tiles()
-    shapes = __import__("batchgemm_amd").TILE_SHAPES
-    seen = set()
-    for mat, tm, tn, kb, ke in items:
-        bm, bn = shapes[tiles[mat]]
-        assert 0 <= tm * bm < M[mat] and 0 <= tn * bn < N[mat]
-        key = (mat, tm, tn, kb)
-        assert key not in seen
-        seen.add(key)
-    expect = sum(-(-int(m) // shapes[t][0]) * -(-int(n) // shapes[t][1])
-                 for m, n, t in zip(M, N, tiles) if m > 0 and n > 0)
-    assert len({(a, b, c) for a, b, c, _ in seen}) == expect
+def _cover(plan, M, N, sample=40):
+    """Every output element of every matrix covered exactly once per K slice;
+    no item outside its matrix (exact on a sample, area totals on all)."""
+    import batchgemm_amd
+    shapes = batchgemm_amd.TILE_SHAPES
+    items = plan.tiled_items()
+    ks = plan.ksplit()
+    area = np.zeros(len(M), np.int64)
+    rng = np.random.default_rng(0)
+    pick = set(rng.choice(len(M), min(sample, len(M)), replace=False).tolist())
+    grids = {b: np.zeros((int(ks[b]), int(M[b]), int(N[b])), np.int16) for b in pick}
+    for mat, t, tm, tn, sl in items:
+        bm, bn = shapes[t]
+        r0, c0 = tm * bm, tn * bn
+        assert 0 <= r0 < M[mat] and 0 <= c0 < N[mat] and 0 <= sl < ks[mat]
+        r1, c1 = min(r0 + bm, M[mat]), min(c0 + bn, N[mat])
+        area[mat] += (r1 - r0) * (c1 - c0)
+        if mat in grids:
+            grids[mat][sl, r0:r1, c0:c1] += 1
+    for b in range(len(M)):
+        assert area[b] == int(M[b]) * int(N[b]) * int(ks[b]) if M[b] and N[b] else area[b] == 0
+    for b, g in grids.items():
+        assert g.size == 0 or (g.min() == 1 and g.max() == 1), b
 
 
 @pytest.mark.parametrize("policy", ["auto", "reference", "wang", "li", "fixed"])
@@ -58,9 +66,10 @@ def test_host_plan_covers_every_tile_once(bg, policy):
                                       w.shapes_unbalanced(512, seed=1),
                                       w.shapes_loguniform(300, seed=2),
                                       w.read_dataset(DATA, 256)]):
-        p = bg.Plan(M, N, K, host_only=True, policy=policy, tlp_thres=6000)
-        _cover(p, M, N)
-        assert p.n_items > 0 and 1 <= p.n_launches <= 6
+        for edges in (1, -1):
+            p = bg.Plan(M, N, K, host_only=True, policy=policy, tlp_thres=6000, edge_tiles=edges)
+            _cover(p, M, N)
+            assert p.n_items > 0 and 1 <= p.n_launches <= 6
 
 
 def test_host_plan_zero_sized_matrices(bg):
@@ -129,3 +138,25 @@ def test_reference_policy_follows_reference_planner(bg, orc):
         ref = orc.ref_plan(M[:B], N[:B], K[:B], 6000)
         to_tile = [0, 1, 1, 2, 2]
         assert np.array_equal(p.tiles(), np.array([to_tile[t] for t in ref["t"]]))
+
+
+def test_host_plan_split_k_and_edges_cover(bg):
+    from batchgemm_amd import workloads as w
+    M, N, K = w.shapes_unbalanced(300, seed=3)
+    for prec in ("fp32", "bf16x3"):
+        p = bg.Plan(M, N, K, host_only=True, prec=prec, force_ksplit=3)
+        assert (p.ksplit() > 1).any()
+        _cover(p, M, N)
+
+
+def test_edge_tiles_cut_padding(bg):
+    """A 130 x 190 matrix on 128x128 tiles: its last row strip (2 rows) and
+    column strip (62 columns) go to 64-wide tiles, cutting the padded area."""
+    shapes = bg.TILE_SHAPES
+    def padded(p):
+        return sum(shapes[t][0] * shapes[t][1] for _, t, _, _, _ in p.tiled_items())
+    on = bg.Plan([130], [190], [64], host_only=True, policy="fixed", fixed_tile="128x128")
+    off = bg.Plan([130], [190], [64], host_only=True, policy="fixed", fixed_tile="128x128", edge_tiles=-1)
+    assert padded(on) < padded(off) and padded(off) == 4 * 128 * 128
+    # 64x64 corner, 128x64 right strip, 64x128 bottom strip, 128x128 interior
+    assert sorted({int(t) for _, t, _, _, _ in on.tiled_items()}) == [2, 3, 4, 5]
