@@ -34,7 +34,7 @@ EXPORTS = [
     "vbg_abi_version", "vbg_status_string", "vbg_last_error", "vbg_opts_default",
     "vbg_plan_create", "vbg_plan_destroy", "vbg_sgemm_vbatched", "vbg_sgemm_vbatched_oneshot",
     "vbg_plan_query", "vbg_ref_plan", "vbg_grouped_plan_create", "vbg_grouped_plan_destroy",
-    "vbg_grouped_gemm_fwd", "vbg_grouped_gemm_bwd",
+    "vbg_grouped_gemm_fwd", "vbg_grouped_gemm_bwd", "vbg_plan_launch_times",
 ]
 
 
@@ -60,6 +60,7 @@ class vbg_opts(ctypes.Structure):
         ("force_ksplit", ctypes.c_int),
         ("tile_group", ctypes.c_int),
         ("edge_tiles", ctypes.c_int),
+        ("timing_slots", ctypes.c_int),
     ]
 
 
@@ -92,6 +93,7 @@ def lib():
                                              ctypes.c_char, ctypes.c_int, ctypes.c_float, vp, vp,
                                              ctypes.c_float, vp, vp]
     L.vbg_plan_query.argtypes = [vp, ctypes.c_int, ip, ctypes.c_int]
+    L.vbg_plan_launch_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
     L.vbg_ref_plan.argtypes = [ctypes.c_int, ip, ip, ip, ctypes.c_int, ip, ip, ip,
                                ctypes.POINTER(ctypes.c_int64)]
     L.vbg_grouped_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ctypes.c_int,

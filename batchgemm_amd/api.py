@@ -22,6 +22,10 @@ from . import _lib
 from ._lib import PREC, check, default_opts, lib
 
 
+def _lib_tile_shapes():
+    return _lib.TILE_SHAPES
+
+
 def _iarr(x):
     a = np.ascontiguousarray(np.asarray(x, dtype=np.int32))
     return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
@@ -132,6 +136,40 @@ class Plan:
         check(lib().vbg_plan_query(self._p, 4, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                    len(out)), "vbg_plan_query")
         return out[:5 * n].reshape(n, 5)
+
+    KERNEL_KINDS = ("scale pre-pass", "fp32 mfma", "fp32 sequential", "split precision", "bf16",
+                    "split-K reduction")
+
+    def kernels(self) -> np.ndarray:
+        """Kernels a run issues, in order, as rows (kind, tile id, first item,
+        item count, grid); kind indexes ``KERNEL_KINDS``, items index
+        ``items()`` / ``tiled_items()``."""
+        out = np.zeros(5 * 16, np.int32)
+        n = check(lib().vbg_plan_query(self._p, 7, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       len(out)), "vbg_plan_query")
+        return out[:n].reshape(-1, 5)
+
+    def kernel_times(self) -> np.ndarray:
+        """Mean ms per kernel (rows of ``kernels()``) over the recorded runs of
+        a plan made with ``timing_slots > 0``; waits for those runs."""
+        out = np.zeros(16, np.float32)
+        n = check(lib().vbg_plan_launch_times(self._p, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_float)), len(out)), "vbg_plan_launch_times")
+        return out[:n].astype(np.float64)
+
+    def item_flops(self) -> np.ndarray:
+        """Algorithmic flops of every work item (2 * rows * cols * k of its
+        part of the matrix, padding excluded), in ``items()`` order."""
+        shapes = np.asarray(_lib_tile_shapes(), np.int64)
+        ti = self.tiled_items().astype(np.int64)
+        kr = self.items().astype(np.int64)
+        if len(ti) == 0:
+            return np.zeros(0)
+        bm, bn = shapes[ti[:, 1], 0], shapes[ti[:, 1], 1]
+        mat = ti[:, 0]
+        rows = np.minimum(bm, self.M[mat].astype(np.int64) - ti[:, 2] * bm)
+        cols = np.minimum(bn, self.N[mat].astype(np.int64) - ti[:, 3] * bn)
+        return 2.0 * rows * cols * (kr[:, 4] - kr[:, 3])
 
     # -- execution
     def run(self, dA: torch.Tensor, dB: torch.Tensor, dC: torch.Tensor, alpha=1.0, beta=0.0,

@@ -4,6 +4,7 @@
 // exits, ROCm/include/util.h:58-64).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -59,7 +60,40 @@ struct vbg_plan {
     unsigned* counters;
   };
   std::vector<DevLaunch> dl;
+  // Kernel timing (opts.timing_slots > 0): `slots` rings of rows + 1 events.
+  std::vector<hipEvent_t> ev;
+  int slots = 0;
+  int64_t runs = 0;
 };
+
+namespace {
+
+bool has_prepass(const vbg::HostPlan& hp) {
+  return hp.prec == VBG_PREC_FP16_SCALED || hp.prec == VBG_PREC_FP8_E4M3;
+}
+
+// Kernels a run issues, in order: [scale pre-pass], one per launch, [split-K reduction].
+int kernel_rows(const vbg::HostPlan& hp) {
+  return (has_prepass(hp) ? 1 : 0) + (int)hp.launches.size() + (hp.red_chunks.empty() ? 0 : 1);
+}
+
+int family_kind(int fam) {
+  switch (fam) {
+    case vbg::FAM_FP32: return 1;
+    case vbg::FAM_SEQ: return 2;
+    case vbg::FAM_SPLIT: return 3;
+    case vbg::FAM_BF16: return 4;
+    default: return -1;
+  }
+}
+
+void destroy_events(vbg_plan* p) {
+  for (hipEvent_t e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  p->ev.clear();
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -203,6 +237,18 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     }
     for (size_t l = 0; l < hp.launches.size(); ++l)
       p->dl.push_back({(vbg_item*)(base + oi[l]), (int*)(base + oq[l]), (unsigned*)(base + oc[l])});
+    if (o.timing_slots > 0) {
+      p->slots = o.timing_slots;
+      p->ev.assign((size_t)p->slots * (kernel_rows(hp) + 1), nullptr);
+      for (auto& ev : p->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) {
+          int rc2 = hip_fail(e, "hipEventCreate");
+          vbg_plan_destroy(p);
+          return rc2;
+        }
+      }
+    }
   }
   *plan = p;
   return VBG_OK;
@@ -211,6 +257,7 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
 int vbg_plan_destroy(vbg_plan* p) {
   if (!p) return VBG_OK;
   int rc = VBG_OK;
+  destroy_events(p);
   if (p->dbuf) {
     hipError_t e = hipFree(p->dbuf);
     if (e != hipSuccess) rc = hip_fail(e, "hipFree(plan)");
@@ -236,10 +283,30 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
   if (dev != p->device) return fail(VBG_ERR_PLAN_MISMATCH, "plan was created on another device");
   hipStream_t s = (hipStream_t)stream;
 
-  if (hp.prec == VBG_PREC_FP16_SCALED || hp.prec == VBG_PREC_FP8_E4M3) {
+  // Timing plans bracket every kernel with events of this run's ring slot.
+  hipEvent_t* ev = nullptr;
+  int row = 0;
+  if (p->slots > 0) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    e = hipStreamIsCapturing(s, &cs);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamIsCapturing");
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(VBG_ERR_NOT_SUPPORTED, "a timing plan cannot be captured in a graph");
+    ev = &p->ev[(size_t)(p->runs % p->slots) * (kernel_rows(hp) + 1)];
+    e = hipEventRecord(ev[0], s);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+  }
+  auto mark = [&]() -> int {
+    if (!ev) return VBG_OK;
+    hipError_t er = hipEventRecord(ev[++row], s);
+    return er == hipSuccess ? VBG_OK : hip_fail(er, "hipEventRecord");
+  };
+
+  if (has_prepass(hp)) {
     int r = vbg_launch_scales(hp.prec, hp.batch, hp.ta, hp.tb, p->d_desc, p->d_chunks,
                               (int)hp.chunks.size(), dA, dB, p->d_amax, p->d_scales, s);
     if (r) return fail(VBG_ERR_HIP, std::string("scale pre-pass launch: ") + hipGetErrorString(hipGetLastError()));
+    if (int rc = mark()) return rc;
   }
   for (size_t l = 0; l < hp.launches.size(); ++l) {
     const vbg::Launch& L = hp.launches[l];
@@ -270,14 +337,40 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
     }
     if (r == -1) return fail(VBG_ERR_NOT_SUPPORTED, "tile/precision combination not built");
     if (r != 0) return fail(VBG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(hipGetLastError()));
+    if (int rc = mark()) return rc;
   }
   if (!hp.red_chunks.empty()) {
     int r = vbg_launch_splitk_reduce(p->d_red, (int)hp.red_chunks.size(), p->d_desc, p->d_ws,
                                      p->d_ws_off, dC, alpha, beta,
                                      hp.prec == VBG_PREC_BF16_OUT_BF16, s);
     if (r != 0) return fail(VBG_ERR_HIP, std::string("split-K reduction launch: ") + hipGetErrorString(hipGetLastError()));
+    if (int rc = mark()) return rc;
   }
+  if (ev) ++p->runs;
   return VBG_OK;
+}
+
+int vbg_plan_launch_times(const vbg_plan* p, float* ms, int n) {
+  if (!p) return fail(VBG_ERR_INVALID_ARG, "null plan");
+  if (p->slots <= 0 || p->runs == 0)
+    return fail(VBG_ERR_PLAN_MISMATCH, "plan has no timing slots or no recorded run");
+  const int rows = kernel_rows(p->hp);
+  if (!ms || n < rows) return fail(VBG_ERR_INVALID_ARG, "out too small");
+  const int64_t nrec = std::min<int64_t>(p->runs, p->slots);
+  std::vector<double> acc(rows, 0.0);
+  for (int64_t r = p->runs - nrec; r < p->runs; ++r) {
+    const hipEvent_t* ev = &p->ev[(size_t)(r % p->slots) * (rows + 1)];
+    hipError_t e = hipEventSynchronize(ev[rows]);
+    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    for (int k = 0; k < rows; ++k) {
+      float t = 0.0f;
+      e = hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+      if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+      acc[k] += t;
+    }
+  }
+  for (int k = 0; k < rows; ++k) ms[k] = (float)(acc[k] / (double)nrec);
+  return rows;
 }
 
 int vbg_sgemm_vbatched_oneshot(int batch, const int* M, const int* N, const int* K, const int* lda,
@@ -329,11 +422,11 @@ int vbg_plan_query(const vbg_plan* p, int what, int* out, int out_len) {
         for (const auto& it : L.items) {
           const vbg_mat_desc& d = hp.desc[it.mat];
           int kb = 0, ke = d.K;
-          if (d.ksplit > 1) {
-            const int bk = 32;
-            const int kt = (d.K + bk - 1) / bk, per = (kt + d.ksplit - 1) / d.ksplit;
-            kb = std::min(d.K, it.kslice * per * bk);
-            ke = std::min(d.K, kb + per * bk);
+          if (d.ksplit > 1) {  // same rule as the kernels' vbg_k_range
+            const int al = VBG_KSLICE_ALIGN;
+            const int kt = (d.K + al - 1) / al, per = (kt + d.ksplit - 1) / d.ksplit;
+            kb = std::min(d.K, it.kslice * per * al);
+            ke = std::min(d.K, kb + per * al);
           }
           out[w++] = it.mat;
           out[w++] = it.tm;
@@ -341,6 +434,25 @@ int vbg_plan_query(const vbg_plan* p, int what, int* out, int out_len) {
           out[w++] = kb;
           out[w++] = ke;
         }
+      return w;
+    }
+    case 7: {
+      const int rows = kernel_rows(hp);
+      if (!out || out_len < 5 * rows) return fail(VBG_ERR_INVALID_ARG, "out too small");
+      int w = 0, first = 0;
+      auto put = [&](int kind, int tile, int f, int cnt, int grid) {
+        out[w++] = kind;
+        out[w++] = tile;
+        out[w++] = f;
+        out[w++] = cnt;
+        out[w++] = grid;
+      };
+      if (has_prepass(hp)) put(0, -1, 0, 0, (int)hp.chunks.size());
+      for (const auto& L : hp.launches) {
+        put(family_kind(L.family), L.tile, first, (int)L.items.size(), L.grid);
+        first += (int)L.items.size();
+      }
+      if (!hp.red_chunks.empty()) put(5, -1, 0, 0, (int)hp.red_chunks.size());
       return w;
     }
     default: return fail(VBG_ERR_INVALID_ARG, "unknown query");

@@ -52,3 +52,5 @@ static const int kTileBN[NTILES] = {16, 32, 64, 64, 128, 128};
 // Work-queue count of the XCD-aware scheduler: MI355X has 8 XCDs and
 // dispatches consecutive workgroup ids round-robin across them.
 #define VBG_NUM_XCD 8
+// Granularity of split-K slices (elements of K); a multiple of every tile's BK.
+#define VBG_KSLICE_ALIGN 64

@@ -342,8 +342,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     int64_t ws = 0;
     for (int b = 0; b < batch; ++b) {
       if (M[b] == 0 || N[b] == 0) continue;
-      const int bk = fam == FAM_BF16 ? 64 : kTileBK[tile[b]];
-      const int ks = cdiv(K[b], bk);
+      const int ks = cdiv(K[b], VBG_KSLICE_ALIGN);
       int S = 1;
       if (o.force_ksplit > 0) {
         S = std::min(o.force_ksplit, ks);

@@ -212,18 +212,21 @@ __device__ __forceinline__ void vbg_persistent_loop(const vbg_args& a, Body&& bo
   }
 }
 
-// k range of an item's slice: slices cover [0, K) in chunks that are whole
-// multiples of BK, the last one taking the remainder.
+// k range of an item's slice: slices cover [0, K) in whole multiples of
+// VBG_KSLICE_ALIGN (a multiple of every kernel's BK, so the slice boundaries
+// do not depend on which tile shape computes a part of the matrix), the last
+// one taking the remainder.
 struct KRange {
   int kb, ke;
 };
 template <int BK>
 __device__ __forceinline__ KRange vbg_k_range(int K, int ksplit, int slice) {
+  static_assert(VBG_KSLICE_ALIGN % BK == 0, "k slices must be whole k-tiles");
   if (ksplit <= 1) return {0, K};
-  const int ktiles = (K + BK - 1) / BK;
-  const int per = (ktiles + ksplit - 1) / ksplit;
-  const int kb = min(K, slice * per * BK);
-  return {kb, min(K, kb + per * BK)};
+  const int chunks = (K + VBG_KSLICE_ALIGN - 1) / VBG_KSLICE_ALIGN;
+  const int per = (chunks + ksplit - 1) / ksplit;
+  const int kb = min(K, slice * per * VBG_KSLICE_ALIGN);
+  return {kb, min(K, kb + per * VBG_KSLICE_ALIGN)};
 }
 
 // Where an item's tile goes: the matrix's C with alpha*scale and beta, or --

@@ -41,6 +41,54 @@ def time_launches(fn, steps, warmup):
     return s.elapsed_time(e) / steps  # ms
 
 
+KERNEL_TEMPLATE = {1: "Fp32MfmaTile", 2: "SeqTile", 3: "SplitTile", 4: "Bf16Tile"}
+
+
+def kernel_name(bg, prec, kind, tile):
+    """Readable name of a plan kernel: family template and tile shape (the
+    symbols rocprof lists are vbg_gemm_kernel / vbg_split_kernel /
+    vbg_bf16_kernel instantiated on these templates)."""
+    if kind == 0:
+        return "vbg_amax_kernel+vbg_scales_kernel"
+    if kind == 5:
+        return "vbg_splitk_reduce"
+    bm, bn = bg.TILE_SHAPES[tile]
+    if kind == 1 and bm == bn and bm <= 32:
+        return f"Fp32SmallTile<{bm}>"
+    return f"{KERNEL_TEMPLATE[kind]}<{bm},{bn}>" + (f"[{prec}]" if kind == 3 else "")
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE, WRITE_SIZE, calibrated on a copy kernel) and kept in
+    profiles/traffic_unbalanced_b2048.txt as lines `<kernel> <bytes>`;
+    None when that kernel was not measured."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "traffic_unbalanced_b2048.txt")
+    try:
+        with open(path) as f:
+            for ln in f:
+                parts = ln.split()
+                if len(parts) == 2 and parts[0] == kernel and not ln.startswith("#"):
+                    return int(float(parts[1]))
+    except OSError:
+        pass
+    return None
+
+
+def per_matrix_library(vb, M, N, K, alpha, beta, steps, warmup):
+    """The per-matrix-launch comparator: one torch.addmm (rocBLAS/hipBLASLt
+    FP32 GEMM) per matrix, as the reference's hipBLAS loop
+    (ROCm/default/gemm.cuh:64-85), on the same column-major buffers."""
+    import torch
+    mats = [(vb.a(b), vb.b(b), vb.c(b)) for b in range(len(M))]
+
+    def loop():
+        for a, b, c in mats:
+            torch.addmm(c, a, b, beta=beta, alpha=alpha, out=c)
+    return time_launches(loop, steps, warmup)
+
+
 def cpu_baseline(M, N, K, alpha, beta, budget_s=12.0):
     """Host per-matrix SGEMM loop (oracle/liboracle.so, OpenMP across matrices)
     on a bounded sample of the same workload: matrices in random order (keeping
@@ -48,7 +96,9 @@ def cpu_baseline(M, N, K, alpha, beta, budget_s=12.0):
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import oracle as orc
     rng = np.random.default_rng(123)
-    order = rng.permutation(len(M))
+    # the whole workload in random order, repeated (at most 4 passes) until
+    # about budget_s seconds of CPU work
+    order = np.concatenate([rng.permutation(len(M)) for _ in range(4)])
     done_flops, done_t, used = 0, 0.0, 0
     chunk = 64
     while done_t < budget_s and used < len(order):
@@ -65,7 +115,8 @@ def cpu_baseline(M, N, K, alpha, beta, budget_s=12.0):
     return {
         "value": round(done_flops / done_t / 1e9, 2), "unit": "GFLOP/s",
         "cores": orc.num_threads(), "kind": "port",
-        "sample": f"{used} of {len(M)} matrices of the same workload in random order "
+        "sample": f"{used} matrices ({used / len(M):.2f} passes over the {len(M)}) of the same "
+                  f"workload in random order "
                   f"({done_flops / 1e9:.1f} GFLOP, {done_t:.1f} s): host per-matrix FP32 SGEMM, "
                   f"OpenMP across matrices",
     }
@@ -106,6 +157,11 @@ def extras(bg, w, vb, M, N, K, alpha, beta, flops, fp32_value, args):
                      "rel_err_vs_fp64": float(f"{err:.3g}")}
         log(f"[bench] {prec}: {ms:.3f} ms, {out[prec]['gflops']} GFLOP/s, rel err {err:.3g}")
     out["bf16x3"]["speedup_vs_fp32"] = round(out["bf16x3"]["gflops"] / out["fp32"]["gflops"], 3)
+    lib_ms = per_matrix_library(vb, M, N, K, alpha, beta, max(3, args.steps // 4), 1)
+    out["per_matrix_library_fp32"] = {
+        "what": "torch.addmm per matrix (rocBLAS/hipBLASLt), 2048 launches",
+        "ms": round(lib_ms, 3), "gflops": round(flops / (lib_ms * 1e-3) / 1e9, 1)}
+    log(f"[bench] per-matrix library loop: {lib_ms:.3f} ms")
     # grouped forward + backward
     from batchgemm_amd.grouped import GroupedPlan
     m_sizes = w.zipf_groups(128, 32768, 1.2, seed=args.seed)
@@ -140,6 +196,28 @@ def extras(bg, w, vb, M, N, K, alpha, beta, flops, fp32_value, args):
     return out
 
 
+def traffic_probe(bg, vb, M, N, K, alpha, beta, args):
+    """Kernels for HBM-traffic counters: a plain copy of 1 GiB (reads 1 GiB,
+    writes 1 GiB: the scale against which FETCH_SIZE / WRITE_SIZE are read),
+    then the workload's plan, each dispatch on its own."""
+    import torch
+    n = (1 << 30) // 4
+    src = torch.empty(n, dtype=torch.float32, device="cuda").uniform_()
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    plan = bg.Plan(M, N, K, prec=args.prec, policy=args.policy)
+    for _ in range(args.steps):
+        plan.run(vb.dA, vb.dB, vb.dC, alpha, beta)
+        torch.cuda.synchronize()
+    print(json.dumps({"traffic_probe": True, "copy_bytes": n * 4, "prec": args.prec,
+                      "steps": args.steps, "kernels": [
+                          kernel_name(bg, args.prec, int(k), int(t))
+                          for k, t, *_ in plan.kernels()]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +230,10 @@ def main():
     ap.add_argument("--policy", default="auto")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the bf16x3 and grouped fwd+bwd measurements")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: one 1 GiB device copy (the calibration "
+                         "kernel of known traffic), then --steps runs of the workload's plan; "
+                         "no timing, no extras")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -170,8 +252,11 @@ def main():
     log(f"[bench] workload: 2048 matrices, {flops / 1e9:.1f} GFLOP, prec={args.prec}")
     vb = w.VBatch(M, N, K).fill("normal", seed=args.seed, c_dist="uniform")
     torch.cuda.synchronize()
+    if args.traffic_probe:
+        traffic_probe(bg, vb, M, N, K, alpha, beta, args)
+        return
     t0 = time.perf_counter()
-    plan = bg.Plan(M, N, K, prec=args.prec, policy=args.policy)
+    plan = bg.Plan(M, N, K, prec=args.prec, policy=args.policy, timing_slots=args.steps)
     plan_ms = (time.perf_counter() - t0) * 1e3
     log(f"[bench] plan: {plan.n_items} items, {plan.n_launches} launches, grid {plan.grid}, "
         f"{plan_ms:.1f} ms")
@@ -181,20 +266,26 @@ def main():
     value = flops / (ms * 1e-3) / 1e9
     log(f"[bench] {ms:.3f} ms/step  {value:.1f} GFLOP/s")
 
-    # Dominant kernel: the launch holding the 128x128 FP32 MFMA tiles (the
-    # large matrices).  Its matrices are planned alone with the same tile and
-    # timed with HIP events on the launch stream.
-    tiles = plan.tiles()
-    dom_tile = int(np.bincount(tiles[tiles >= 0], weights=(2.0 * M * N * K)[tiles >= 0]).argmax())
-    sel = np.where(tiles == dom_tile)[0]
-    sub = bg.Plan(M[sel], N[sel], K[sel], prec=args.prec, policy="fixed", fixed_tile=dom_tile)
-    dA, dB, dC = vb.dA[sel], vb.dB[sel], vb.dC[sel]
-    sub_ms = time_launches(lambda: sub.run(dA, dB, dC, alpha, beta), args.steps, args.warmup)
-    sub_flops = w.flops(M[sel], N[sel], K[sel])
-    achieved = sub_flops / (sub_ms * 1e-3)
+    # Per-kernel durations of the timed steps (HIP events recorded by the plan
+    # around each of its kernels, on the launch stream) and each kernel's
+    # algorithmic flops (2*rows*cols*k of its items, padding excluded).
+    kt = plan.kernels()
+    kms = plan.kernel_times()
+    iflops = plan.item_flops()
+    rows = []
+    for (kind, tile, first, cnt, grid), t in zip(kt, kms):
+        f = float(iflops[first:first + cnt].sum()) if kind in (1, 2, 3, 4) else 0.0
+        rows.append({"kernel": kernel_name(bg, args.prec, int(kind), int(tile)), "items": int(cnt),
+                     "grid": int(grid), "ms": round(float(t), 4), "gflop": round(f / 1e9, 3),
+                     "tflops": round(f / (t * 1e-3) / 1e12, 1) if f and t > 0 else None})
+    dom = max(range(len(rows)), key=lambda r: rows[r]["gflop"])
+    dom_ms = float(kms[dom]); dom_flops = rows[dom]["gflop"] * 1e9
+    achieved = dom_flops / (dom_ms * 1e-3)
     peak = PEAK[args.prec]
-    log(f"[bench] dominant launch: tile {bg.TILE_SHAPES[dom_tile]}, {len(sel)} matrices, "
-        f"{sub_ms:.3f} ms, {achieved / 1e12:.1f} TFLOP/s")
+    log(f"[bench] kernels: " + "; ".join(f"{r['kernel']} {r['ms']} ms {r['gflop']} GFLOP" for r in rows))
+    log(f"[bench] dominant kernel {rows[dom]['kernel']}: {dom_ms:.3f} ms, "
+        f"{achieved / 1e12:.1f} TFLOP/s")
+    traffic = read_traffic(rows[dom]["kernel"])
 
     line = {
         "metric": "batched GFLOP/s (2*sum(MNK)/t), unbalanced variable-size FP32 batched GEMM",
@@ -216,9 +307,9 @@ def main():
                    "policy": args.policy, "plan_ms": round(plan_ms, 2)},
         "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 2),
                      "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": f"vbg_gemm_kernel<{bg.TILE_SHAPES[dom_tile]}>",
-                     "launch_ms": round(sub_ms, 4), "launch_gflop": round(sub_flops / 1e9, 3)},
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "kernel": rows[dom]["kernel"], "launch_ms": round(dom_ms, 4),
+                     "launch_gflop": rows[dom]["gflop"], "kernels": rows},
         "cpu_baseline": None,
     }
     if args.prec == "fp32" and not args.no_extras:

@@ -101,6 +101,10 @@ typedef struct vbg_opts {
   int edge_tiles;       /* cover a last tile row / column of <= 64 rows / columns with 64-wide
                            tiles (less padded work): > 0 on, < 0 off, 0 = per-family default
                            (on for FP32 and BF16 operands, off for bf16x3 / FP16 / FP8) */
+  int timing_slots;     /* > 0: every run records HIP events around each of its kernels on the
+                           run's stream, keeping the last `timing_slots` runs for
+                           vbg_plan_launch_times (all work still runs; a timing plan must not
+                           be captured in a graph).  0 = off */
 } vbg_opts;
 
 typedef struct vbg_plan vbg_plan;
@@ -156,8 +160,24 @@ int vbg_sgemm_vbatched_oneshot(int batch, const int* M, const int* N, const int*
  *   5 split-K parts per matrix (out[batch])
  *   6 work items as (mat, tile id, tile_m, tile_n, k slice) x n (out[5*n]); tile_m / tile_n
  *     count in units of that item's tile shape
+ *   7 kernel table, one row of (kind, tile id, first item, item count, grid) per kernel a run
+ *     issues, in issue order (out[5*rows]); kind 0 = scale pre-pass (FP16 / FP8), 1 = FP32
+ *     MFMA tiles, 2 = FP32 sequential-fma tiles, 3 = split-precision tiles, 4 = BF16 tiles,
+ *     5 = split-K reduction; items index the lists of queries 4 and 6
  * Returns the number of ints written (or the scalar for 0, 1, 3). */
 int vbg_plan_query(const vbg_plan* plan, int what, int* out, int out_len);
+
+/*
+ * Per-kernel durations of a timing plan (opts.timing_slots > 0), in the row
+ * order of query 7: ms[r] = mean over the recorded runs (at most timing_slots,
+ * the most recent) of the time between the HIP events before and after kernel
+ * r on the run's stream.  Waits for those events.  Counterpart of the
+ * reference's hipEvent timing of its launches (ROCm/framework/gemm.cuh:
+ * 240-266), resolved per kernel.  Returns the number of rows written, or a
+ * negative status (VBG_ERR_PLAN_MISMATCH for a plan without timing or with no
+ * recorded run).
+ */
+int vbg_plan_launch_times(const vbg_plan* plan, float* ms, int n);
 
 /*
  * Reference planner, restated (ROCm/framework/gemm.cuh:121-229): initial tile

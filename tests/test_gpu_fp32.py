@@ -135,3 +135,29 @@ def test_fp32_split_k_forced(bg, orc, gpu, tile):
                             fixed_tile=tile, force_ksplit=3)
     assert (plan.ksplit() > 1).sum() >= 8
     oracle_check(orc, vb, 1.5, 0.5, C0)
+
+
+def test_kernel_timing_plan(bg, orc, gpu):
+    """A timing plan brackets each kernel with events: one positive duration
+    per kernel of query 7, results unchanged, refused under graph capture."""
+    import torch
+    from batchgemm_amd import workloads as w
+    from batchgemm_amd._lib import VbgError
+    say("[fp32] kernel timing")
+    M, N, K = w.shapes_unbalanced(96, seed=9)
+    vb = w.VBatch(M, N, K).fill("normal", seed=9, c_dist="uniform")
+    C0 = vb.C.cpu().numpy()
+    plan = bg.Plan(M, N, K, timing_slots=3, force_ksplit=2)
+    plan.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
+    torch.cuda.synchronize()
+    oracle_check(orc, vb, 1.5, 0.5, C0)
+    for _ in range(4):
+        plan.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
+    t = plan.kernel_times()
+    kt = plan.kernels()
+    assert len(t) == len(kt) and kt[-1, 0] == 5 and (t > 0).all(), (t, kt)
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with pytest.raises(VbgError):
+        with torch.cuda.graph(g, stream=s):
+            plan.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5, stream=s)

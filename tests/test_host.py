@@ -160,3 +160,36 @@ def test_edge_tiles_cut_padding(bg):
     assert padded(on) < padded(off) and padded(off) == 4 * 128 * 128
     # 64x64 corner, 128x64 right strip, 64x128 bottom strip, 128x128 interior
     assert sorted({int(t) for _, t, _, _, _ in on.tiled_items()}) == [2, 3, 4, 5]
+
+
+def test_kernel_table_and_item_flops(bg):
+    """Query 7 lists the kernels of a run in order; per-item algorithmic flops
+    sum to 2*sum(MNK) (padding excluded, split-K slices partition K)."""
+    from batchgemm_amd import workloads as w
+    from batchgemm_amd._lib import VbgError
+    M, N, K = w.shapes_unbalanced(300, seed=4)
+    total = 2.0 * np.sum(M.astype(np.float64) * N * K)
+    for prec, ks in (("fp32", 0), ("fp32", 3), ("fp16", 0), ("bf16x3", 5), ("fp32_seq", 0)):
+        p = bg.Plan(M, N, K, host_only=True, prec=prec, force_ksplit=ks)
+        kt = p.kernels()
+        kinds = list(kt[:, 0])
+        assert (kinds[0] == 0) == (prec == "fp16")
+        assert (kinds[-1] == 5) == (ks > 1)
+        body = kt[[k not in (0, 5) for k in kinds]]
+        assert len(body) == p.n_launches
+        assert body[0, 2] == 0 and body[:, 3].sum() == p.n_items
+        assert np.array_equal(body[1:, 2], np.cumsum(body[:, 3])[:-1])
+        fl = p.item_flops()
+        assert abs(fl.sum() - total) <= 1e-9 * total, prec
+        # k slices of one output tile partition [0, K)
+        ti, kr = p.tiled_items(), p.items()
+        spans = {}
+        for (mat, t, tm, tn, sl), (_, _, _, kb, ke) in zip(ti, kr):
+            spans.setdefault((mat, t, tm, tn), []).append((kb, ke))
+        for (mat, *_), sp in spans.items():
+            sp.sort()
+            assert sp[0][0] == 0 and sp[-1][1] == K[mat]
+            assert all(a[1] == b[0] for a, b in zip(sp, sp[1:]))
+        with pytest.raises(VbgError) as e:
+            p.kernel_times()
+        assert e.value.code == -5
