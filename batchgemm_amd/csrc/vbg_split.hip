@@ -42,17 +42,29 @@ struct Mode;
 // and the second stage costs registers), FP16/FP8 with 2.  WAVES: the
 // register-allocation target handed to the compiler (bf16x3's 3 keeps its
 // accumulators in AGPRs, the allocation that was measured fastest).
+// LDS_STAGES: LDS buffers (2: the next tile is stored while the current one
+// is read; 1: one buffer and a second barrier per k-tile, half the LDS).
+// bf16x3 measured on the unbalanced B=2048 batch: waves 3 / LDS 2 (occupancy
+// 2, no spill) 211.6 TFLOP/s; waves 3 / LDS 1 (occupancy 3, 40 B/lane spill)
+// 213.6 -- within noise, so occupancy is not what limits this kernel.
+#ifndef VBG_BF16X3_WAVES
+#define VBG_BF16X3_WAVES 3
+#endif
+#ifndef VBG_BF16X3_LDS_STAGES
+#define VBG_BF16X3_LDS_STAGES 2
+#endif
 template <>
 struct Mode<MODE_BF16X3> {
-  static constexpr int EB = 2, PLANES = 2, STAGES = 1, WAVES = 3;
+  static constexpr int EB = 2, PLANES = 2, STAGES = 1, WAVES = VBG_BF16X3_WAVES,
+                       LDS_STAGES = VBG_BF16X3_LDS_STAGES;
 };
 template <>
 struct Mode<MODE_FP16> {
-  static constexpr int EB = 2, PLANES = 1, STAGES = 2, WAVES = 2;
+  static constexpr int EB = 2, PLANES = 1, STAGES = 2, WAVES = 2, LDS_STAGES = 2;
 };
 template <>
 struct Mode<MODE_FP8> {
-  static constexpr int EB = 1, PLANES = 1, STAGES = 2, WAVES = 2;
+  static constexpr int EB = 1, PLANES = 1, STAGES = 2, WAVES = 2, LDS_STAGES = 2;
 };
 
 // LDS layout per operand plane: [mn][k] with k contiguous, rows of BK*EB bytes
@@ -74,7 +86,9 @@ struct SplitTile {
   }
   static constexpr int PLANE_A = BM * PITCH, PLANE_B = BN * PITCH;
   static constexpr int STAGE = PLANES * (PLANE_A + PLANE_B);  // bytes
-  static constexpr int LDS_FLOATS = 2 * STAGE / 4;
+  static constexpr int LDS_STAGES = Mode<MODE>::LDS_STAGES;
+  static_assert(LDS_STAGES == 2 || Mode<MODE>::STAGES == 1, "two register stages need two LDS buffers");
+  static constexpr int LDS_FLOATS = LDS_STAGES * STAGE / 4;
   static constexpr int WTM = BM / 2, WTN = BN / 2;
   static constexpr int TM = WTM / 32, TN = WTN / 32;
   static_assert(TM >= 1 && TN >= 1, "split tile needs >= 64x64");
@@ -251,12 +265,13 @@ struct SplitTile {
     }
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
-      const int cur = kt & 1;
+      const int cur = LDS_STAGES == 2 ? (kt & 1) : 0;
       const bool more = kt + 1 < nk;
       if (more) load(la0, lb0, kt + 1);
       compute(lds + cur * STAGE, [](int) {});
+      if (LDS_STAGES == 1) __syncthreads();  // every wave is done reading the buffer
       if (more) {
-        char* other = lds + (cur ^ 1) * STAGE;
+        char* other = lds + (LDS_STAGES == 2 ? (cur ^ 1) : 0) * STAGE;
         store<LA>(la0, other, PLANE_A, sA);
         store<LB>(lb0, other + PLANES * PLANE_A, PLANE_B, sB);
       }

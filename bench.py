@@ -139,8 +139,9 @@ def rel_err_sample(w, vb, M, N, K, alpha, beta, C0, idx):
 
 def extras(bg, w, vb, M, N, K, alpha, beta, flops, fp32_value, args):
     """bf16x3 on the same batch (GFLOP/s, ratio to FP32, error vs FP64 on a
-    sample), and the grouped BF16 forward+backward (G=128, K=2048, N=1408,
-    Zipf(1.2) group sizes summing to 32768)."""
+    sample); all four precision paths on outlier-stressed inputs of the same
+    shapes; the per-matrix library loop; the grouped BF16 forward+backward
+    (G=128, K=2048, N=1408, Zipf(1.2) group sizes summing to 32768)."""
     import torch
     out = {}
     rng = np.random.default_rng(1)
@@ -157,6 +158,25 @@ def extras(bg, w, vb, M, N, K, alpha, beta, flops, fp32_value, args):
                      "rel_err_vs_fp64": float(f"{err:.3g}")}
         log(f"[bench] {prec}: {ms:.3f} ms, {out[prec]['gflops']} GFLOP/s, rel err {err:.3g}")
     out["bf16x3"]["speedup_vs_fp32"] = round(out["bf16x3"]["gflops"] / out["fp32"]["gflops"], 3)
+    # Precision paths on the same shapes with N(0,1) inputs and 1% of the
+    # entries scaled by 1e3 (stresses the per-matrix scaling); error of the
+    # sampled matrices against FP64.
+    vb.fill("normal", seed=args.seed + 1, outlier_frac=0.01, outlier_scale=1e3, c_dist="uniform")
+    prec_out = {}
+    for prec in ("fp32", "bf16x3", "fp16", "fp8"):
+        p = bg.Plan(M, N, K, prec=prec)
+        ms = time_launches(lambda: p.run(vb.dA, vb.dB, vb.dC, alpha, beta), max(3, args.steps // 2),
+                           args.warmup)
+        C0 = vb.C.clone()
+        p.run(vb.dA, vb.dB, vb.dC, alpha, beta)
+        torch.cuda.synchronize()
+        err = rel_err_sample(w, vb, M, N, K, alpha, beta, C0, idx)
+        prec_out[prec] = {"gflops": round(flops / (ms * 1e-3) / 1e9, 1), "ms": round(ms, 4),
+                          "rel_err_vs_fp64": float(f"{err:.3g}")}
+        log(f"[bench] outlier inputs {prec}: {ms:.3f} ms, {prec_out[prec]['gflops']} GFLOP/s, "
+            f"rel err {err:.3g}")
+        del p
+    out["precision_outliers"] = {"inputs": "N(0,1), 1% of entries x1e3, C ~ U[-1,1]", **prec_out}
     lib_ms = per_matrix_library(vb, M, N, K, alpha, beta, max(3, args.steps // 4), 1)
     out["per_matrix_library_fp32"] = {
         "what": "torch.addmm per matrix (rocBLAS/hipBLASLt), 2048 launches",

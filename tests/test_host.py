@@ -193,3 +193,52 @@ def test_kernel_table_and_item_flops(bg):
         with pytest.raises(VbgError) as e:
             p.kernel_times()
         assert e.value.code == -5
+
+
+def test_driver_gflop_count_matches_reference_logs(bg):
+    """The driver's GFLOP count is the reference's sum(2MNK + 2MN) (the
+    numbers ROCm/framework/log-test-verify3 prints for B = 8..256)."""
+    import json
+    from batchgemm_amd import workloads as w
+    from batchgemm_amd.driver import reference_gflop
+    logs = json.load(open(os.path.join(HERE, "golden", "reference_logs.json")))
+    for B, row in zip(logs["batches"], logs["rocm_framework"]["rows"]):
+        M, N, K = w.read_dataset(DATA, B)
+        assert f"{reference_gflop(M, N, K):f}" == row["gflop"], B
+
+
+def test_autotune_keys_and_cache(bg, tmp_path):
+    """Batches of one distribution share coarse keys; the JSON cache
+    round-trips; an unknown key falls back to the default heuristic."""
+    from batchgemm_amd import autotune as at
+    batches = at.batching_workload(12, seed=3)
+    for M, N, K in batches:
+        assert 256 <= len(M) <= 8192 and M.min() >= 8 and max(M.max(), N.max(), K.max()) <= 1024
+    keys = {at.distribution_key(M, N, K, "fp32") for M, N, K in batches}
+    assert len(keys) <= 7            # one per log2(B) bucket at most
+    assert at.distribution_key(*batches[0], "fp32") != at.distribution_key(*batches[0], "bf16x3")
+    path = str(tmp_path / "tune.json")
+    t = at.Autotuner(cache_path=path, tune_missing=False)
+    k = at.distribution_key(*batches[0], "fp32")
+    assert t.choice(k) == ("auto", {})
+    t.table[k] = {"choice": "auto:ge64", "ms": {}, "gain": 1.1}
+    t.save()
+    t2 = at.Autotuner(cache_path=path, tune_missing=False)
+    assert t2.choice(k) == ("auto:ge64", {"tile_mask": 0b111100})
+    p = t2.plan(*batches[0], prec="fp32", host_only=True)
+    assert set(int(x) for x in p.tiles()) <= {2, 3, 4, 5}
+    names = [n for n, _ in at.CANDIDATES]
+    assert len(set(names)) == len(names) and names[0] == at.DEFAULT
+
+
+def test_driver_sequential_model():
+    """The host replay of the reference tiles' fma sequence equals the
+    closed form while sums stay below 2^24 and departs from it above."""
+    from batchgemm_amd.driver import sequential_fp32
+    K = np.array([16, 64, 128, 8] + [128] * 252)
+    v = sequential_fp32(K, 11)
+    exact = np.array([(i + 3) + 11 * int(K[i]) * (i + 1) * (i + 2) for i in range(len(K))], np.float64)
+    small = exact < 2 ** 24
+    assert small[:4].all() and np.array_equal(v[small].astype(np.float64), exact[small])
+    assert (v[~small].astype(np.float64) != exact[~small]).any()
+    assert np.allclose(v, exact, rtol=2e-4)
