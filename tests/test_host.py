@@ -242,3 +242,13 @@ def test_driver_sequential_model():
     assert small[:4].all() and np.array_equal(v[small].astype(np.float64), exact[small])
     assert (v[~small].astype(np.float64) != exact[~small]).any()
     assert np.allclose(v, exact, rtol=2e-4)
+
+
+def test_opts_struct_matches_header(bg):
+    """The ctypes vbg_opts has the header's fields in the header's order."""
+    src = open(os.path.join(ROOT, "include", "vbgemm.h")).read()
+    body = re.search(r"typedef struct vbg_opts \{(.*?)\} vbg_opts;", src, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"\b(?:int|unsigned)\s+([a-z_]+)\s*;", body)
+    assert fields == [f for f, _ in bg._lib.vbg_opts._fields_]
+    assert ctypes.sizeof(bg._lib.vbg_opts) == 4 * len(fields)
