@@ -28,6 +28,22 @@ __device__ __forceinline__ v4bf bf16x4(v4f x) {
   const v2bf b = __builtin_convertvector(v2f{x[2], x[3]}, v2bf);
   return __builtin_shufflevector(a, b, 0, 1, 2, 3);
 }
+// bf16x3 split of 4 floats: hi = bf16_rne(x), lo = bf16_rne(x - hi).  hi is
+// converted once, two values per v_cvt_pk_bf16_f32, and recovered as floats
+// from the packed word itself (low half: shift left 16; high half: mask), so
+// each pair costs 2 conversions + 2 integer ops + 2 subtractions.
+__device__ __forceinline__ void bf16x3_split4(v4f x, v4bf& hi, v4bf& lo) {
+  const v2bf h01 = __builtin_convertvector(v2f{x[0], x[1]}, v2bf);
+  const v2bf h23 = __builtin_convertvector(v2f{x[2], x[3]}, v2bf);
+  const unsigned u01 = __builtin_bit_cast(unsigned, h01), u23 = __builtin_bit_cast(unsigned, h23);
+  const v2f r01 = {x[0] - __builtin_bit_cast(float, u01 << 16),
+                   x[1] - __builtin_bit_cast(float, u01 & 0xffff0000u)};
+  const v2f r23 = {x[2] - __builtin_bit_cast(float, u23 << 16),
+                   x[3] - __builtin_bit_cast(float, u23 & 0xffff0000u)};
+  hi = __builtin_shufflevector(h01, h23, 0, 1, 2, 3);
+  lo = __builtin_shufflevector(__builtin_convertvector(r01, v2bf), __builtin_convertvector(r23, v2bf),
+                               0, 1, 2, 3);
+}
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef _Float16 v4h __attribute__((ext_vector_type(4)));
 
@@ -53,36 +69,50 @@ struct Mode;
 #ifndef VBG_BF16X3_LDS_STAGES
 #define VBG_BF16X3_LDS_STAGES 2
 #endif
+#ifndef VBG_BF16X3_BK
+#define VBG_BF16X3_BK 32
+#endif
+#ifndef VBG_FP16_BK
+#define VBG_FP16_BK 32
+#endif
+#ifndef VBG_FP16_STAGES
+#define VBG_FP16_STAGES 2
+#endif
 template <>
 struct Mode<MODE_BF16X3> {
   static constexpr int EB = 2, PLANES = 2, STAGES = 1, WAVES = VBG_BF16X3_WAVES,
-                       LDS_STAGES = VBG_BF16X3_LDS_STAGES;
+                       LDS_STAGES = VBG_BF16X3_LDS_STAGES, BK = VBG_BF16X3_BK;
 };
 template <>
 struct Mode<MODE_FP16> {
-  static constexpr int EB = 2, PLANES = 1, STAGES = 2, WAVES = 2, LDS_STAGES = 2;
+  static constexpr int EB = 2, PLANES = 1, STAGES = VBG_FP16_STAGES, WAVES = 2, LDS_STAGES = 2,
+                       BK = VBG_FP16_BK;
 };
 template <>
 struct Mode<MODE_FP8> {
-  static constexpr int EB = 1, PLANES = 1, STAGES = 2, WAVES = 2, LDS_STAGES = 2;
+  static constexpr int EB = 1, PLANES = 1, STAGES = VBG_FP16_STAGES, WAVES = 2, LDS_STAGES = 2,
+                       BK = VBG_FP16_BK;
 };
 
 // LDS layout per operand plane: [mn][k] with k contiguous, rows of BK*EB bytes
-// split into 4 units (16 B for BF16/FP16, 8 B for FP8: one MFMA operand read);
-// unit u of row r sits at unit u ^ ((r / RPB) % 4), RPB = 256 / row bytes, so
-// the 32 rows an MFMA operand read touches hit distinct LDS banks without row
-// padding.  Lane half h of an MFMA wave owns k in [h*BK/2, (h+1)*BK/2); step s
-// of the 32x32x16 MFMA takes 8 of them, k = h*BK/2 + 8s + [0, 8).
+// split into NU = BK/8 units of 8 k (16 B for BF16/FP16, 8 B for FP8: one MFMA
+// operand read); unit u of row r sits at unit u ^ ((r / RPB) % NU), RPB =
+// 256 / row bytes (at least 1), so the rows an MFMA operand read touches hit
+// distinct LDS banks without row padding.  Lane half h of an MFMA wave owns k
+// in [h*BK/2, (h+1)*BK/2); step s of the 32x32x16 MFMA takes 8 of them,
+// k = h*BK/2 + 8s + [0, 8).
 template <int BM, int BN, int MODE, bool TA, bool TB>
 struct SplitTile {
-  static constexpr int BK = 32;
+  static constexpr int BK = Mode<MODE>::BK;
+  static_assert(BK == 32 || BK == 64, "k-tile depth 32 or 64");
   static constexpr int MODE_ID = MODE;
   static constexpr int EB = Mode<MODE>::EB, PLANES = Mode<MODE>::PLANES;
   static constexpr int PITCH = BK * EB;  // bytes per LDS row (no padding: swizzled)
-  static constexpr int UNIT = PITCH / 4;  // bytes per MFMA operand read
-  static constexpr int RPB = 256 / PITCH;
+  static constexpr int UNIT = 8 * EB;    // bytes per MFMA operand read
+  static constexpr int NU = PITCH / UNIT;
+  static constexpr int RPB = PITCH >= 256 ? 1 : 256 / PITCH;
   __device__ __forceinline__ static int swz(int row, int byte) {
-    return row * PITCH + ((((byte / UNIT) ^ (row / RPB)) & 3) * UNIT) + (byte % UNIT);
+    return row * PITCH + ((((byte / UNIT) ^ (row / RPB)) & (NU - 1)) * UNIT) + (byte % UNIT);
   }
   static constexpr int PLANE_A = BM * PITCH, PLANE_B = BN * PITCH;
   static constexpr int STAGE = PLANES * (PLANE_A + PLANE_B);  // bytes
@@ -106,8 +136,8 @@ struct SplitTile {
       const v4f x = rr[v];
       char* dst = plane0 + swz(row, 4 * kq * EB);
       if (MODE == MODE_BF16X3) {
-        const v4bf hi = bf16x4(x);
-        const v4bf lo = bf16x4(x - __builtin_convertvector(hi, v4f));  // x - hi is exact
+        v4bf hi, lo;
+        bf16x3_split4(x, hi, lo);  // x - hi is exact
         *(v4bf*)dst = hi;
         *(v4bf*)(dst + plane_bytes) = lo;
       } else if (MODE == MODE_FP16) {
