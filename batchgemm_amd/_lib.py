@@ -28,6 +28,7 @@ PREC = {"fp32": 0, "bf16x3": 1, "fp16": 2, "fp8": 3, "fp32_seq": 4, "bf16": 5, "
 POLICY = {"auto": 0, "reference": 1, "fixed": 2, "wang": 3, "li": 4}
 TILES = {"16x16": 0, "32x32": 1, "64x64": 2, "128x64": 3, "64x128": 4, "128x128": 5}
 TILE_SHAPES = [(16, 16), (32, 32), (64, 64), (128, 64), (64, 128), (128, 128)]
+TILE_MIXED = 6  # launch tile id of a mixed launch (query 7): the shape is per item
 
 # Every symbol include/vbgemm.h declares (checked by the CPU test-suite).
 EXPORTS = [
@@ -61,6 +62,7 @@ class vbg_opts(ctypes.Structure):
         ("tile_group", ctypes.c_int),
         ("edge_tiles", ctypes.c_int),
         ("timing_slots", ctypes.c_int),
+        ("mixed_tiles", ctypes.c_int),
     ]
 
 

@@ -10,7 +10,9 @@ variants of this library's own cost model, are candidates:
 - ``auto``                   the default heuristic (cost model, all tiles)
 - ``auto`` + tile set        ``tile_mask``: no 128x128 / only >= 64x64 / no 16x16
 - ``auto`` + thresholds      ``big_tile_min_mn`` 256 or 512; split-K off;
-                             edge tiles off; tile grouping off
+                             edge tiles off / on; tile grouping off
+- ``auto`` + launch shape    one mixed launch for the 64..128 tiles, or one
+                             launch per tile shape
 - ``reference``              the reference planner, wavefront_thres 2500 / 6000 / 20000
 - ``wang`` / ``li``          TLP thresholds 65536 / 262144
 
@@ -52,6 +54,9 @@ CANDIDATES = [
     ("auto:big512", {"big_tile_min_mn": 512}),
     ("auto:nosplitk", {"split_k_min": -1}),
     ("auto:noedges", {"edge_tiles": -1}),
+    ("auto:edges", {"edge_tiles": 1}),
+    ("auto:mixed", {"mixed_tiles": 1}),
+    ("auto:separate", {"mixed_tiles": -1}),
     ("auto:group1", {"tile_group": 1}),
     ("reference:2500", {"policy": "reference", "wavefront_thres": 2500}),
     ("reference:6000", {"policy": "reference", "wavefront_thres": 6000}),

@@ -408,7 +408,7 @@ int vbg_plan_query(const vbg_plan* p, int what, int* out, int out_len) {
       for (const auto& L : hp.launches)
         for (const auto& it : L.items) {
           out[w++] = it.mat;
-          out[w++] = L.tile;
+          out[w++] = it.tile;
           out[w++] = it.tm;
           out[w++] = it.tn;
           out[w++] = it.kslice;

@@ -290,11 +290,46 @@ struct SeqTile {
   }
 };
 
+// Register-allocation target: tiles may name a minimum number of waves per
+// SIMD (amdgpu_waves_per_eu); 1 leaves the compiler free.
+template <class T, class = void>
+struct min_waves {
+  static constexpr int value = 1;
+};
+template <class T>
+struct min_waves<T, decltype((void)T::MIN_WAVES)> {
+  static constexpr int value = T::MIN_WAVES;
+};
+
 template <class Tile>
-__global__ void __launch_bounds__(VBG_THREADS) vbg_gemm_kernel(vbg_args a) {
+__global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(min_waves<Tile>::value)))
+vbg_gemm_kernel(vbg_args a) {
   __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
   vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
 }
+
+// Mixed launch: one persistent grid over items of the 64x64 .. 128x128
+// shapes, each item naming its own (wave-uniform) shape.  LDS is sized for
+// the largest; registers are those of the largest body.
+template <bool TA, bool TB>
+struct Fp32Mixed {
+  using Sq128 = Fp32MfmaTile<128, 128, 32, TA, TB>;
+  using Tall = Fp32MfmaTile<128, 64, 32, TA, TB>;
+  using Wide = Fp32MfmaTile<64, 128, 32, TA, TB>;
+  using Sq64 = Fp32MfmaTile<64, 64, 32, TA, TB>;
+  static constexpr int LDS_FLOATS = Sq128::LDS_FLOATS;
+  static constexpr int MIN_WAVES = 2;  // two workgroups per CU, as the 128x128 body alone
+  static_assert(Sq128::LDS_FLOATS >= Tall::LDS_FLOATS && Sq128::LDS_FLOATS >= Sq64::LDS_FLOATS,
+                "the 128x128 body must need the most LDS");
+  __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
+    switch (it.tile) {
+      case T128: Sq128::run(a, it, lds); break;
+      case T128x64: Tall::run(a, it, lds); break;
+      case T64x128: Wide::run(a, it, lds); break;
+      default: Sq64::run(a, it, lds); break;
+    }
+  }
+};
 
 }  // namespace
 
@@ -340,6 +375,7 @@ template <bool TA, bool TB> using F32T64 = Fp32MfmaTile<64, 64, 32, TA, TB>;
 template <bool TA, bool TB> using F32T128x64 = Fp32MfmaTile<128, 64, 32, TA, TB>;
 template <bool TA, bool TB> using F32T64x128 = Fp32MfmaTile<64, 128, 32, TA, TB>;
 template <bool TA, bool TB> using F32T128 = Fp32MfmaTile<128, 128, 32, TA, TB>;
+template <bool TA, bool TB> using F32Mixed = Fp32Mixed<TA, TB>;
 
 static int fp32_dispatch(int tile, bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s,
                          int* occ) {
@@ -350,6 +386,7 @@ static int fp32_dispatch(int tile, bool ta, bool tb, const vbg_args* a, int grid
     case T128x64: return dispatch_tt<F32T128x64>(ta, tb, a, grid, s, occ);
     case T64x128: return dispatch_tt<F32T64x128>(ta, tb, a, grid, s, occ);
     case T128: return dispatch_tt<F32T128>(ta, tb, a, grid, s, occ);
+    case T_MIXED: return dispatch_tt<F32Mixed>(ta, tb, a, grid, s, occ);
     default: return -1;
   }
 }

@@ -14,8 +14,9 @@ struct vbg_mat_desc {
 // One unit of work: a (matrix, tile_m, tile_n, k-slice) quadruple.
 struct vbg_item {
   int mat;
-  int tm, tn;
-  int kslice;
+  int tm, tn;     // tile coordinates in units of the item's tile shape
+  short kslice;   // split-K slice (0 when K is not split)
+  short tile;     // tile id of this item (T16 .. T128)
 };
 
 // A slice of one stored operand for the per-matrix amax pre-pass (FP16 / FP8):
@@ -46,6 +47,9 @@ struct vbg_args {
 
 // Tile ids (mirror vbg_tile in include/vbgemm.h).
 enum { T16 = 0, T32 = 1, T64 = 2, T128x64 = 3, T64x128 = 4, T128 = 5, NTILES = 6 };
+// Launch tile id of a mixed launch: one persistent kernel whose items are of
+// the shapes 64x64, 128x64, 64x128 and 128x128, each carrying its own tile id.
+enum { T_MIXED = 6 };
 static const int kTileBM[NTILES] = {16, 32, 64, 128, 64, 128};
 static const int kTileBN[NTILES] = {16, 32, 64, 64, 128, 128};
 

@@ -377,9 +377,20 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     int tm0, tm1, tn0, tn1;  // tile ranges in the part's own tile units
   };
   std::vector<std::vector<Part>> parts(NTILES);
-  // default (0): on for the FP32 and BF16 families; the split family measured
-  // faster without (its extra small-tile launches cost more than the padding)
-  const bool edges = fam != FAM_SEQ && (o.edge_tiles > 0 || (o.edge_tiles == 0 && fam != FAM_SPLIT));
+  // Mixed launch: the parts of every shape from 64x64 to 128x128 share one
+  // persistent launch, so edge strips and medium matrices fill the big tiles'
+  // queues instead of running as launches of their own after them.  Measured
+  // on the unbalanced B=2048 batch (TFLOP/s, mixed+edges / separate+edges /
+  // separate without edges): FP32 96 / 109 / 106, bf16x3 156 / 209 / 218,
+  // FP16 206 / 192 / 202, FP8 206 / 192 / 202 -- the combined kernel loses
+  // the 128x128 body's accumulator-in-AGPR allocation that FP32 and bf16x3
+  // run fastest with.  Default (0): mixed for FP16 / FP8; edge tiles where
+  // the launches are mixed and for the FP32 and BF16 families.
+  const bool split_scaled = prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3;
+  const bool mixed = (fam == FAM_FP32 || fam == FAM_SPLIT) &&
+                     (o.mixed_tiles > 0 || (o.mixed_tiles == 0 && split_scaled));
+  const bool edges = fam != FAM_SEQ && (o.edge_tiles > 0 ||
+                                        (o.edge_tiles == 0 && (mixed || fam == FAM_FP32 || fam == FAM_BF16)));
   for (int b = 0; b < batch; ++b) {
     if (M[b] == 0 || N[b] == 0) continue;
     const int t = tile[b];
@@ -401,21 +412,44 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     if (half_m && half_n) parts[T64].push_back({b, 2 * rows, 2 * rows + 1, 2 * cols, 2 * cols + 1});
   }
 
-  // 2b. one persistent launch per tile shape; 3. LPT over XCD queues.
-  for (int t = 0; t < NTILES; ++t) {
-    std::vector<Part>& ps = parts[t];
-    if (ps.empty()) continue;
-    auto pcost = [&](const Part& x) {
-      return (double)(x.tm1 - x.tm0) * (x.tn1 - x.tn0) * hp.desc[x.b].ksplit * slice_cost(t, x.b);
+  // 2b. one persistent launch per tile shape (or one for all mixed shapes);
+  // 3. LPT over XCD queues.
+  struct TPart {
+    Part p;
+    int t;
+  };
+  std::vector<std::vector<TPart>> groups;
+  std::vector<int> group_tile;
+  {
+    std::vector<TPart> mix;
+    for (int t = 0; t < NTILES; ++t) {
+      if (parts[t].empty()) continue;
+      const bool into_mix = mixed && t >= T64;
+      std::vector<TPart> g;
+      for (const Part& x : parts[t]) (into_mix ? mix : g).push_back({x, t});
+      if (!into_mix) {
+        groups.push_back(std::move(g));
+        group_tile.push_back(t);
+      }
+    }
+    if (!mix.empty()) {
+      groups.push_back(std::move(mix));
+      group_tile.push_back(T_MIXED);
+    }
+  }
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    std::vector<TPart>& ps = groups[gi];
+    auto pcost = [&](const TPart& x) {
+      return (double)(x.p.tm1 - x.p.tm0) * (x.p.tn1 - x.p.tn0) * hp.desc[x.p.b].ksplit * slice_cost(x.t, x.p.b);
     };
     Launch L;
-    L.tile = t;
+    L.tile = group_tile[gi];
     L.family = fam;
     // longest parts first, ties by matrix index for determinism
-    std::stable_sort(ps.begin(), ps.end(), [&](const Part& x, const Part& y) { return pcost(x) > pcost(y); });
-    std::vector<std::vector<Part>> qm(hp.nq);
+    std::stable_sort(ps.begin(), ps.end(), [&](const TPart& x, const TPart& y) { return pcost(x) > pcost(y); });
+    std::vector<std::vector<TPart>> qm(hp.nq);
     std::vector<double> load(hp.nq, 0.0);
-    for (const Part& x : ps) {
+    for (const TPart& x : ps) {
       const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       qm[q].push_back(x);
       load[q] += pcost(x);
@@ -424,22 +458,24 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     L.q_begin.push_back(0);
     for (int q = 0; q < hp.nq; ++q) {
       // per-item cost decreasing; a part's tiles stay together
-      std::stable_sort(qm[q].begin(), qm[q].end(), [&](const Part& x, const Part& y) {
-        return slice_cost(t, x.b) > slice_cost(t, y.b);
+      std::stable_sort(qm[q].begin(), qm[q].end(), [&](const TPart& x, const TPart& y) {
+        return slice_cost(x.t, x.p.b) > slice_cost(y.t, y.p.b);
       });
       // tiles in runs of `grp` tile rows, column by column: the ~64 items an
       // XCD runs at once form a grp x (64/grp) block of the matrix, touching
       // grp A panels and 64/grp B panels instead of 64/tn rows x tn columns
       const int grp = o.tile_group > 0 ? o.tile_group : 8;
-      for (const Part& x : qm[q]) {
+      for (const TPart& tp : qm[q]) {
+        const Part& x = tp.p;
         for (int sl = 0; sl < hp.desc[x.b].ksplit; ++sl)
           for (int i0 = x.tm0; i0 < x.tm1; i0 += grp)
             for (int j = x.tn0; j < x.tn1; ++j)
-              for (int i = i0; i < std::min(x.tm1, i0 + grp); ++i) L.items.push_back({x.b, i, j, sl});
+              for (int i = i0; i < std::min(x.tm1, i0 + grp); ++i)
+                L.items.push_back({x.b, i, j, (short)sl, (short)tp.t});
       }
       L.q_begin.push_back((int)L.items.size());
     }
-    const int per_cu = occ(fam, t);
+    const int per_cu = occ(fam, L.tile);
     int grid = o.grid_wgs > 0 ? o.grid_wgs : cus * per_cu;
     grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)L.items.size()));
     L.grid = grid;

@@ -18,7 +18,7 @@ namespace vbg {
 enum Family { FAM_FP32 = 0, FAM_SEQ = 1, FAM_SPLIT = 2, FAM_BF16 = 3 };
 
 struct Launch {
-  int tile = 0;                 // tile id (T16 .. T128)
+  int tile = 0;                 // tile id (T16 .. T128), or T_MIXED (tile per item)
   int family = FAM_FP32;
   std::vector<vbg_item> items;  // in queue order
   std::vector<int> q_begin;     // nq + 1

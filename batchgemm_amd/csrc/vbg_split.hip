@@ -106,6 +106,7 @@ struct SplitTile {
   static constexpr int BK = Mode<MODE>::BK;
   static_assert(BK == 32 || BK == 64, "k-tile depth 32 or 64");
   static constexpr int MODE_ID = MODE;
+  static constexpr int WAVES = Mode<MODE>::WAVES;
   static constexpr int EB = Mode<MODE>::EB, PLANES = Mode<MODE>::PLANES;
   static constexpr int PITCH = BK * EB;  // bytes per LDS row (no padding: swizzled)
   static constexpr int UNIT = 8 * EB;    // bytes per MFMA operand read
@@ -318,8 +319,31 @@ struct SplitTile {
   }
 };
 
+// Mixed launch: items of the 64x64 .. 128x128 shapes in one persistent grid,
+// each naming its own (wave-uniform) shape; LDS sized for the largest.
+template <int MODE, bool TA, bool TB>
+struct SplitMixed {
+  using Sq128 = SplitTile<128, 128, MODE, TA, TB>;
+  using Tall = SplitTile<128, 64, MODE, TA, TB>;
+  using Wide = SplitTile<64, 128, MODE, TA, TB>;
+  using Sq64 = SplitTile<64, 64, MODE, TA, TB>;
+  static constexpr int MODE_ID = MODE;
+  static constexpr int WAVES = 2;  // two workgroups per CU, as the 128x128 body alone
+  static constexpr int LDS_FLOATS = Sq128::LDS_FLOATS;
+  static_assert(Sq128::LDS_FLOATS >= Tall::LDS_FLOATS && Sq128::LDS_FLOATS >= Sq64::LDS_FLOATS,
+                "the 128x128 body must need the most LDS");
+  __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
+    switch (it.tile) {
+      case T128: Sq128::run(a, it, lds); break;
+      case T128x64: Tall::run(a, it, lds); break;
+      case T64x128: Wide::run(a, it, lds); break;
+      default: Sq64::run(a, it, lds); break;
+    }
+  }
+};
+
 template <class Tile>
-__global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(Mode<Tile::MODE_ID>::WAVES)))
+__global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(Tile::WAVES)))
 vbg_split_kernel(vbg_args a) {
   __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
   vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
@@ -413,6 +437,14 @@ static int dispatch_tt(bool ta, bool tb, const vbg_args* a, int grid, hipStream_
 }
 
 template <int MODE>
+static int dispatch_mixed(bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  if (!ta && !tb) return run_or_occ<SplitMixed<MODE, false, false>>(a, grid, s, occ);
+  if (!ta && tb) return run_or_occ<SplitMixed<MODE, false, true>>(a, grid, s, occ);
+  if (ta && !tb) return run_or_occ<SplitMixed<MODE, true, false>>(a, grid, s, occ);
+  return run_or_occ<SplitMixed<MODE, true, true>>(a, grid, s, occ);
+}
+
+template <int MODE>
 static int dispatch_tile(int tile, bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s,
                          int* occ) {
   switch (tile) {
@@ -420,6 +452,7 @@ static int dispatch_tile(int tile, bool ta, bool tb, const vbg_args* a, int grid
     case T128x64: return dispatch_tt<128, 64, MODE>(ta, tb, a, grid, s, occ);
     case T64x128: return dispatch_tt<64, 128, MODE>(ta, tb, a, grid, s, occ);
     case T128: return dispatch_tt<128, 128, MODE>(ta, tb, a, grid, s, occ);
+    case T_MIXED: return dispatch_mixed<MODE>(ta, tb, a, grid, s, occ);
     default: return -1;
   }
 }

@@ -44,7 +44,10 @@ __device__ __forceinline__ vbg_mat_desc vbg_udesc(const vbg_mat_desc& d) {
 }
 
 __device__ __forceinline__ vbg_item vbg_uitem(const vbg_item& it) {
-  return {vbg_u(it.mat), vbg_u(it.tm), vbg_u(it.tn), vbg_u(it.kslice)};
+  int packed;  // kslice | tile << 16, one scalar register
+  __builtin_memcpy(&packed, &it.kslice, sizeof(packed));
+  packed = vbg_u(packed);
+  return {vbg_u(it.mat), vbg_u(it.tm), vbg_u(it.tn), (short)(packed & 0xffff), (short)(packed >> 16)};
 }
 
 // ---------------------------------------------------------------------------

@@ -52,6 +52,8 @@ def kernel_name(bg, prec, kind, tile):
         return "vbg_amax_kernel+vbg_scales_kernel"
     if kind == 5:
         return "vbg_splitk_reduce"
+    if tile == bg._lib.TILE_MIXED:  # one launch over the 64x64 .. 128x128 shapes
+        return {1: "Fp32Mixed", 3: f"SplitMixed[{prec}]"}.get(kind, f"{KERNEL_TEMPLATE[kind]}<mixed>")
     bm, bn = bg.TILE_SHAPES[tile]
     if kind == 1 and bm == bn and bm <= 32:
         return f"Fp32SmallTile<{bm}>"

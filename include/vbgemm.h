@@ -99,12 +99,16 @@ typedef struct vbg_opts {
                            column (0 = 8; 1 = plain row-major), so the workgroups of one XCD
                            that run consecutive items share A and B panels in its L2 */
   int edge_tiles;       /* cover a last tile row / column of <= 64 rows / columns with 64-wide
-                           tiles (less padded work): > 0 on, < 0 off, 0 = per-family default
-                           (on for FP32 and BF16 operands, off for bf16x3 / FP16 / FP8) */
+                           tiles (less padded work): > 0 on, < 0 off, 0 = default (on for FP32,
+                           BF16 and mixed launches; off for bf16x3) */
   int timing_slots;     /* > 0: every run records HIP events around each of its kernels on the
                            run's stream, keeping the last `timing_slots` runs for
                            vbg_plan_launch_times (all work still runs; a timing plan must not
                            be captured in a graph).  0 = off */
+  int mixed_tiles;      /* > 0: FP32 and split-precision plans run every 64x64 .. 128x128 tile
+                           (interior tiles, edge strips, medium matrices) in one persistent
+                           launch whose items carry their own shape; < 0: one launch per tile
+                           shape; 0 = default (mixed for FP16 / FP8 only) */
 } vbg_opts;
 
 typedef struct vbg_plan vbg_plan;
@@ -161,7 +165,8 @@ int vbg_sgemm_vbatched_oneshot(int batch, const int* M, const int* N, const int*
  *   6 work items as (mat, tile id, tile_m, tile_n, k slice) x n (out[5*n]); tile_m / tile_n
  *     count in units of that item's tile shape
  *   7 kernel table, one row of (kind, tile id, first item, item count, grid) per kernel a run
- *     issues, in issue order (out[5*rows]); kind 0 = scale pre-pass (FP16 / FP8), 1 = FP32
+ *     issues, in issue order (out[5*rows]); tile id 6 = a mixed launch (items of the 64x64 ..
+ *     128x128 shapes, see query 6); kind 0 = scale pre-pass (FP16 / FP8), 1 = FP32
  *     MFMA tiles, 2 = FP32 sequential-fma tiles, 3 = split-precision tiles, 4 = BF16 tiles,
  *     5 = split-K reduction; items index the lists of queries 4 and 6
  * Returns the number of ints written (or the scalar for 0, 1, 3). */

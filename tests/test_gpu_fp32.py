@@ -54,13 +54,18 @@ EDGE_SHAPES = ([1, 2, 15, 16, 17, 33, 63, 64, 65, 127, 129, 200, 31],
                [1, 0, 7, 16, 33, 40, 64, 129, 3, 200, 255, 64, 1])
 
 
+@pytest.mark.parametrize("mixed", [1, -1])
 @pytest.mark.parametrize("tile", ["16x16", "32x32", "64x64", "128x64", "64x128", "128x128"])
 @pytest.mark.parametrize("trans", ["NN", "NT", "TN", "TT"])
-def test_fp32_every_tile_every_transpose(bg, orc, gpu, tile, trans):
-    say(f"[fp32] tile={tile} trans={trans}")
+def test_fp32_every_tile_every_transpose(bg, orc, gpu, tile, trans, mixed):
+    """Every tile shape and transpose, in the mixed launch (items of several
+    shapes in one persistent kernel) and in launches of one shape each."""
+    say(f"[fp32] tile={tile} trans={trans} mixed={mixed}")
     M, N, K = EDGE_SHAPES
-    vb, C0, _ = run_case(bg, orc, M, N, K, trans[0], trans[1], 1.5, 0.5, 1, policy="fixed",
-                         fixed_tile=tile)
+    vb, C0, plan = run_case(bg, orc, M, N, K, trans[0], trans[1], 1.5, 0.5, 1, policy="fixed",
+                            fixed_tile=tile, mixed_tiles=mixed)
+    kt = plan.kernels()
+    assert (kt[:, 1] == 6).any() == (mixed > 0 and tile not in ("16x16", "32x32"))
     oracle_check(orc, vb, 1.5, 0.5, C0)
 
 

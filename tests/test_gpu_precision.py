@@ -158,3 +158,26 @@ def test_split_precision_split_k(bg, orc, gpu, prec):
         c0 = C0[int(vb.offC[b]): int(vb.offC[b]) + M[b] * N[b]].reshape(N[b], M[b]).T.astype(np.float64)
         got = vb.host_flat("C", b).reshape(N[b], M[b]).T
         assert orc.rel_fro(got, 1.5 * emulate(orc, prec, A, B) + 0.5 * c0) <= EMUL_BOUND[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16", "fp8"])
+def test_mixed_launch_equals_per_shape_launches(bg, gpu, prec):
+    """Every element's k order is the same whatever tile shape computes it, so
+    the mixed launch with edge tiles equals one-launch-per-shape without edge
+    tiles bit for bit."""
+    import torch
+    from batchgemm_amd import workloads as w
+    print(f"[mixed] {prec}", flush=True)
+    M, N, K = w.shapes_unbalanced(200, seed=31)
+    vb = w.VBatch(M, N, K).fill("normal", seed=31, outlier_frac=0.01, c_dist="uniform")
+    C0 = vb.C.clone()
+    outs = []
+    for opts in ({"mixed_tiles": 1}, {"mixed_tiles": -1, "edge_tiles": -1}):
+        vb.C.copy_(C0)
+        p = bg.Plan(M, N, K, prec=prec, **opts)
+        p.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
+        torch.cuda.synchronize()
+        outs.append((vb.C.clone(), p.kernels()))
+    (c0, k0), (c1, k1) = outs
+    assert (k0[:, 1] == 6).any() and not (k1[:, 1] == 6).any()
+    assert torch.equal(c0.view(torch.int32), c1.view(torch.int32))
