@@ -62,7 +62,9 @@ struct Mode;
 // is read; 1: one buffer and a second barrier per k-tile, half the LDS).
 // bf16x3 measured on the unbalanced B=2048 batch: waves 3 / LDS 2 (occupancy
 // 2, no spill) 211.6 TFLOP/s; waves 3 / LDS 1 (occupancy 3, 40 B/lane spill)
-// 213.6 -- within noise, so occupancy is not what limits this kernel.
+// 213.6 -- within noise, so occupancy is not what limits this kernel; BK 64
+// (one LDS buffer) 206.7; two register stages (waves 2, accumulators in VGPRs)
+// 207.5 against 217.0 for one stage.
 #ifndef VBG_BF16X3_WAVES
 #define VBG_BF16X3_WAVES 3
 #endif
@@ -78,9 +80,12 @@ struct Mode;
 #ifndef VBG_FP16_STAGES
 #define VBG_FP16_STAGES 2
 #endif
+#ifndef VBG_BF16X3_STAGES
+#define VBG_BF16X3_STAGES 1
+#endif
 template <>
 struct Mode<MODE_BF16X3> {
-  static constexpr int EB = 2, PLANES = 2, STAGES = 1, WAVES = VBG_BF16X3_WAVES,
+  static constexpr int EB = 2, PLANES = 2, STAGES = VBG_BF16X3_STAGES, WAVES = VBG_BF16X3_WAVES,
                        LDS_STAGES = VBG_BF16X3_LDS_STAGES, BK = VBG_BF16X3_BK;
 };
 template <>
