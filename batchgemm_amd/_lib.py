@@ -63,6 +63,7 @@ class vbg_opts(ctypes.Structure):
         ("edge_tiles", ctypes.c_int),
         ("timing_slots", ctypes.c_int),
         ("mixed_tiles", ctypes.c_int),
+        ("consolidate", ctypes.c_int),
     ]
 
 

@@ -58,6 +58,8 @@ CANDIDATES = [
     ("auto:mixed", {"mixed_tiles": 1}),
     ("auto:separate", {"mixed_tiles": -1}),
     ("auto:group1", {"tile_group": 1}),
+    ("auto:noconsolidate", {"consolidate": -1}),
+    ("auto:only128", {"tile_mask": 0b100000}),
     ("reference:2500", {"policy": "reference", "wavefront_thres": 2500}),
     ("reference:6000", {"policy": "reference", "wavefront_thres": 6000}),
     ("reference:20000", {"policy": "reference", "wavefront_thres": 20000}),

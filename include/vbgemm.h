@@ -109,6 +109,9 @@ typedef struct vbg_opts {
                            (interior tiles, edge strips, medium matrices) in one persistent
                            launch whose items carry their own shape; < 0: one launch per tile
                            shape; 0 = default (mixed for FP16 / FP8 only) */
+  int consolidate;      /* AUTO: >= 0 (default) moves the matrices of a tile shape whose launch
+                           would be short into the dominant launch's shape when that is
+                           modelled cheaper than a launch of their own; < 0 off */
 } vbg_opts;
 
 typedef struct vbg_plan vbg_plan;
