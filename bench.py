@@ -230,14 +230,18 @@ def traffic_probe(bg, vb, M, N, K, alpha, beta, args):
     dst.copy_(src)
     torch.cuda.synchronize()
     del src, dst
-    plan = bg.Plan(M, N, K, prec=args.prec, policy=args.policy)
-    for _ in range(args.steps):
-        plan.run(vb.dA, vb.dB, vb.dC, alpha, beta)
-        torch.cuda.synchronize()
-    print(json.dumps({"traffic_probe": True, "copy_bytes": n * 4, "prec": args.prec,
-                      "steps": args.steps, "kernels": [
-                          kernel_name(bg, args.prec, int(k), int(t))
-                          for k, t, *_ in plan.kernels()]}), flush=True)
+    kernels = {}
+    for prec in args.prec.split(","):
+        opts = {"policy": "fixed", "fixed_tile": "128x128", "mixed_tiles": -1} \
+            if args.workload == "uniform" else {"policy": args.policy}
+        plan = bg.Plan(M, N, K, prec=prec, **opts)
+        for _ in range(args.steps):
+            plan.run(vb.dA, vb.dB, vb.dC, alpha, beta)
+            torch.cuda.synchronize()
+        kernels[prec] = [kernel_name(bg, prec, int(k), int(t)) for k, t, *_ in plan.kernels()]
+        del plan
+    print(json.dumps({"traffic_probe": True, "copy_bytes": n * 4, "workload": args.workload,
+                      "steps": args.steps, "kernels": kernels}), flush=True)
 
 
 def main():
@@ -245,7 +249,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3", "fp16", "fp8"])
+    ap.add_argument("--prec", default="fp32",
+                    help="fp32 | bf16x3 | fp16 | fp8 (--traffic-probe: a comma-separated list)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -254,9 +259,18 @@ def main():
                     help="skip the bf16x3 and grouped fwd+bwd measurements")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: one 1 GiB device copy (the calibration "
-                         "kernel of known traffic), then --steps runs of the workload's plan; "
-                         "no timing, no extras")
+                         "kernel of known traffic), then --steps runs of the workload's plan "
+                         "for each --prec; no timing, no extras")
+    ap.add_argument("--workload", default="unbalanced", choices=["unbalanced", "uniform"],
+                    help="--traffic-probe only: the unbalanced B=2048 batch (default) or a "
+                         "uniform 64 x 2048^3 batch on 128x128 tiles")
     args = ap.parse_args()
+    precs = args.prec.split(",")
+    for pr in precs:
+        if pr not in PEAK or pr == "bf16":
+            ap.error(f"unknown --prec {pr}")
+    if len(precs) > 1 and not args.traffic_probe:
+        ap.error("several --prec values only with --traffic-probe")
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the batched GEMM path is single-GPU in this "
@@ -269,9 +283,12 @@ def main():
     from batchgemm_amd import workloads as w
 
     alpha, beta = 1.5, 0.5
-    M, N, K = w.shapes_unbalanced(2048, seed=args.seed)
+    if args.traffic_probe and args.workload == "uniform":
+        M = N = K = np.full(64, 2048, np.int32)
+    else:
+        M, N, K = w.shapes_unbalanced(2048, seed=args.seed)
     flops = w.flops(M, N, K)
-    log(f"[bench] workload: 2048 matrices, {flops / 1e9:.1f} GFLOP, prec={args.prec}")
+    log(f"[bench] workload: {len(M)} matrices, {flops / 1e9:.1f} GFLOP, prec={args.prec}")
     vb = w.VBatch(M, N, K).fill("normal", seed=args.seed, c_dist="uniform")
     torch.cuda.synchronize()
     if args.traffic_probe:
