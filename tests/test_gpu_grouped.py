@@ -45,7 +45,8 @@ def rel(x, r):
     return float(np.linalg.norm(x - r) / (d if d > 0 else 1.0))
 
 
-@pytest.mark.parametrize("m_sizes", [[5, 0, 64, 1, 130, 33, 200, 7], [64, 17, 3, 48]])
+@pytest.mark.parametrize("m_sizes", [[5, 0, 64, 1, 130, 33, 200, 7], [64, 17, 3, 48],
+                                     [1, 2, 16, 0, 9, 3, 16, 1]])
 @pytest.mark.parametrize("KN", [(128, 128), (256, 72), (64, 320)])
 def test_grouped_fwd_bwd_vs_fp64(bg, gpu, m_sizes, KN):
     import torch
@@ -69,6 +70,8 @@ def test_grouped_fwd_bwd_vs_fp64(bg, gpu, m_sizes, KN):
     for g, m in enumerate(m_sizes):  # empty groups: exact zero gradient
         if m == 0:
             assert torch.count_nonzero(dB[g]) == 0
+        elif m <= 16:  # direct small-group dB: exact products, FP32 sums
+            assert rel(dB[g].cpu().numpy(), rB[g]) <= 1e-6, (g, m)
 
 
 def test_grouped_finite_differences_fp64(bg, gpu):
