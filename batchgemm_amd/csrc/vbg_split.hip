@@ -381,12 +381,31 @@ __global__ void __launch_bounds__(VBG_THREADS)
   const float* p = c.op == 0 ? A[c.mat] : B[c.mat];
   const int rows = c.op == 0 ? (ta ? d.K : d.M) : (tb ? d.N : d.K);
   const int ld = c.op == 0 ? d.lda : d.ldb;
+  // Max of |x| over a contiguous run of n floats: 16-byte buffer loads, four
+  // in flight per thread, over the whole float4s [0, n & ~3) (any 4-byte
+  // alignment: buffer loads need no 16-byte alignment); loads that fall past
+  // the run read 0 or in-run values, neither of which changes the max.  The
+  // last n % 4 floats are read one by one.
+  auto run_max = [](const float* q, int n, float m) {
+    const __amdgpu_buffer_rsrc_t rs = vbg_rsrc(q, (uint32_t)n * 4u);
+    const int n4 = n & ~3;
+    for (int j = n4 + (int)threadIdx.x; j < n; j += VBG_THREADS) m = fmaxf(m, fabsf(q[j]));
+    for (int i = 4 * threadIdx.x; i < n4; i += 16 * VBG_THREADS) {
+      v4f x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        x[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rs, (i + u * 4 * VBG_THREADS) * 4, 0, 0));
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        m = fmaxf(fmaxf(m, fmaxf(fabsf(x[u][0]), fabsf(x[u][1]))), fmaxf(fabsf(x[u][2]), fabsf(x[u][3])));
+    }
+    return m;
+  };
   float m = 0.0f;
-  if (ld == rows) {
-    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += VBG_THREADS) m = fmaxf(m, fabsf(p[i]));
-  } else {
-    for (int64_t col = c.begin; col < c.end; ++col)
-      for (int r = threadIdx.x; r < rows; r += VBG_THREADS) m = fmaxf(m, fabsf(p[r + col * ld]));
+  if (ld == rows) {  // the chunk is elements [begin, end) of the packed matrix
+    m = run_max(p + c.begin, (int)(c.end - c.begin), m);
+  } else {           // the chunk is columns [begin, end)
+    for (int64_t col = c.begin; col < c.end; ++col) m = run_max(p + col * ld, rows, m);
   }
   m = block_max(m);
   if (threadIdx.x == 0) atomicMax(&amax[2 * c.mat + c.op], __float_as_uint(m));

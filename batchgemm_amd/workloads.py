@@ -104,7 +104,7 @@ class VBatch:
     """A batch of column-major FP32 matrices A_b (stored rows x cols with
     ld = rows), B_b, C_b in three device pools."""
 
-    def __init__(self, M, N, K, transA="N", transB="N", device="cuda"):
+    def __init__(self, M, N, K, transA="N", transB="N", device="cuda", pad_ab: int = 0):
         self.M = np.asarray(M, np.int32); self.N = np.asarray(N, np.int32)
         self.K = np.asarray(K, np.int32)
         self.batch = len(self.M)
@@ -112,8 +112,9 @@ class VBatch:
         ta, tb = self.transA == "T", self.transB == "T"
         self.a_rows = np.where(ta, self.K, self.M); self.a_cols = np.where(ta, self.M, self.K)
         self.b_rows = np.where(tb, self.N, self.K); self.b_cols = np.where(tb, self.K, self.N)
-        self.lda = np.maximum(1, self.a_rows).astype(np.int32)
-        self.ldb = np.maximum(1, self.b_rows).astype(np.int32)
+        # pad_ab > 0: A and B stored with leading dimension rows + pad_ab
+        self.lda = (np.maximum(1, self.a_rows) + pad_ab).astype(np.int32)
+        self.ldb = (np.maximum(1, self.b_rows) + pad_ab).astype(np.int32)
         self.ldc = np.maximum(1, self.M).astype(np.int32)
         self.offA, nA = _offsets(self.lda.astype(np.int64) * self.a_cols)
         self.offB, nB = _offsets(self.ldb.astype(np.int64) * self.b_cols)

@@ -181,3 +181,34 @@ def test_mixed_launch_equals_per_shape_launches(bg, gpu, prec):
     (c0, k0), (c1, k1) = outs
     assert (k0[:, 1] == 6).any() and not (k1[:, 1] == 6).any()
     assert torch.equal(c0.view(torch.int32), c1.view(torch.int32))
+
+
+@pytest.mark.parametrize("prec", ["fp16", "fp8"])
+@pytest.mark.parametrize("trans", ["NN", "TN"])
+def test_scaled_paths_padded_leading_dimension(bg, orc, gpu, prec, trans):
+    """A and B with lda, ldb past their row counts: the amax pre-pass walks
+    them column by column and must see every element (and no padding)."""
+    import torch
+    from batchgemm_amd.workloads import VBatch
+    say(f"[{prec}] padded ld trans={trans}")
+    M = [64, 100, 17, 130, 3]
+    N = [64, 33, 140, 128, 7]
+    K = [32, 200, 9, 257, 5]
+    vb = VBatch(M, N, K, trans[0], trans[1], pad_ab=3).fill("normal", seed=13, outlier_frac=0.01,
+                                                          c_dist="uniform")
+    # padding holds huge values: an amax that read it would shrink every scale
+    for pool, off, rows, cols, ld in ((vb.A, vb.offA, vb.a_rows, vb.a_cols, vb.lda),
+                                      (vb.B, vb.offB, vb.b_rows, vb.b_cols, vb.ldb)):
+        for b in range(vb.batch):
+            v = pool[int(off[b]): int(off[b]) + int(ld[b]) * int(cols[b])].view(int(cols[b]), int(ld[b]))
+            v[:, int(rows[b]):] = 1e30
+    C0 = vb.C.cpu().numpy()
+    plan = bg.Plan(vb.M, vb.N, vb.K, vb.lda, vb.ldb, vb.ldc, trans[0], trans[1], prec=prec)
+    plan.run(vb.dA, vb.dB, vb.dC, 1.0, 0.5)
+    torch.cuda.synchronize()
+    for b in range(vb.batch):
+        A = op_matrix(vb, "A", b); B = op_matrix(vb, "B", b)
+        c0 = C0[int(vb.offC[b]): int(vb.offC[b]) + M[b] * N[b]].reshape(N[b], M[b]).T.astype(np.float64)
+        got = vb.host_flat("C", b).reshape(N[b], M[b]).T.astype(np.float64)
+        emul = emulate(orc, prec, A, B) + 0.5 * c0
+        assert orc.rel_fro(got, emul) <= EMUL_BOUND[prec], b
