@@ -120,6 +120,20 @@ const int kRefToTile[5] = {T16, T32, T32, T64, T64};
 // tlp_thres -- then per matrix the largest dividing square tile not above it,
 // then demotion from the batch end while the workgroup count is below
 // tlp_thres (the reference's second threshold; one threshold is used here).
+std::vector<int> wang_order(const std::vector<Shape>& sh) {
+  std::vector<int> ord(sh.size());
+  std::iota(ord.begin(), ord.end(), 0);
+  auto cls = [&](int j) {
+    const int m = sh[j].M, n = sh[j].N;
+    if (m % 64 == 0 && n % 64 == 0) return 0;
+    if (m % 32 == 0 && n % 32 == 0) return 1;
+    return 2;
+  };
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return sh[x].K > sh[y].K; });
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cls(x) < cls(y); });
+  return ord;
+}
+
 void choose_wang(const std::vector<Shape>& sh, int thres, std::vector<int>& tile) {
   const int B = (int)sh.size();
   const int ts[3] = {16, 32, 64};
@@ -144,7 +158,13 @@ void choose_wang(const std::vector<Shape>& sh, int thres, std::vector<int>& tile
   }
   int64_t wg = 0;
   for (int j = 0; j < B; ++j) wg += (int64_t)cdiv(sh[j].M, ts[st[j]]) * cdiv(sh[j].N, ts[st[j]]);
-  for (int j = B - 1; j >= 0 && wg < thres; --j) {
+  // The reference plans the batch in the order of ROCm/data/sort.cpp:17-60
+  // (gemm_wang.cuh:33 reads the sorted file): stable by K descending, then
+  // stable by tile class -- M and N both multiples of 64, then of 32, then
+  // the rest; the demotion pass walks that order from its end.
+  const std::vector<int> ord = wang_order(sh);
+  for (int jj = B - 1; jj >= 0 && wg < thres; --jj) {
+    const int j = ord[jj];
     if (st[j] == 0) continue;
     wg += (int64_t)cdiv(sh[j].M, ts[st[j]]) * cdiv(sh[j].N, ts[st[j]]) * 3;
     st[j] -= 1;

@@ -249,3 +249,52 @@ def rel_fro(x, ref) -> float:
     x = np.asarray(x, np.float64).ravel(); ref = np.asarray(ref, np.float64).ravel()
     d = np.linalg.norm(ref)
     return float(np.linalg.norm(x - ref) / (d if d > 0 else 1.0))
+
+
+def wang_order(M, N, K) -> np.ndarray:
+    """Order of ROCm/data/sort.cpp:17-60 (direction 1): stable by K
+    descending, then stable by tile class -- M and N both multiples of 64
+    (class 0), of 32 (class 1), the rest (class 2).  Returns the
+    permutation (indices into the batch)."""
+    M = np.asarray(M, np.int64); N = np.asarray(N, np.int64); K = np.asarray(K, np.int64)
+    cls = np.where((M % 64 == 0) & (N % 64 == 0), 0, np.where((M % 32 == 0) & (N % 32 == 0), 1, 2))
+    return np.lexsort((np.arange(len(M)), -K, cls))  # last key primary; index keeps it stable
+
+
+def wang_plan(M, N, K, thres: int) -> np.ndarray:
+    """Tile index per matrix (0/1/2 = 16x16/32x32/64x64) of the Wang
+    planner, ROCm/tiling/gemm_wang.cuh:104-182 run on the sort.cpp order
+    (gemm_wang.cuh:33 reads the sorted file), with one threshold for both of
+    its TLP tests; workgroup counts round up for sides the tile does not
+    divide (the reference assumes it divides)."""
+    M = [int(x) for x in M]; N = [int(x) for x in N]
+    B = len(M)
+    ts = (16, 32, 64)
+    t = 0
+    while t < 3:
+        tlp = sum((M[j] // ts[t]) * (N[j] // ts[t]) * 128 for j in range(B))
+        if tlp < thres:
+            break
+        t += 1
+    t = min(t, 2)
+    fits = lambda j, k: ts[k] <= M[j] and ts[k] <= N[j] and M[j] % ts[k] == 0 and N[j] % ts[k] == 0
+    st = [0] * B
+    for j in range(B):
+        if fits(j, t):
+            st[j] = t
+        else:
+            for k in range(t):
+                if fits(j, k):
+                    st[j] = k
+    cdiv = lambda a, b: -(-a // b)
+    wg = sum(cdiv(M[j], ts[st[j]]) * cdiv(N[j], ts[st[j]]) for j in range(B))
+    order = wang_order(M, N, K)
+    for jj in range(B - 1, -1, -1):
+        if wg >= thres:
+            break
+        j = int(order[jj])
+        if st[j] == 0:
+            continue
+        wg += cdiv(M[j], ts[st[j]]) * cdiv(N[j], ts[st[j]]) * 3
+        st[j] -= 1
+    return np.array(st, np.int32)

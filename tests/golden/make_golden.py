@@ -1,14 +1,19 @@
-"""Writes tests/golden/reference_logs.json from the reference's run logs.
+"""Writes the golden fixtures of tests/golden from the reference tree:
+
+- reference_logs.json: numbers read from the reference's run logs, with the
+  file:line they come from (the oracle's dataset ingest and FLOP accounting
+  are compared against them);
+- data_MN_K_512_128 and data_MN_K_512_128_sort: the reference's dataset and
+  its sorted first 256 lines, as shipped (ROCm/data/; the sorted file is the
+  output of ROCm/data/sort.cpp, which pins the Wang ordering).
 
 Run in the build container (the reference tree is not present elsewhere):
     python tests/golden/make_golden.py
-The JSON holds only numbers read from the logs, with the file:line they come
-from; tests compare the oracle's dataset ingest and FLOP accounting against
-them.
 """
 import json
 import os
 import re
+import shutil
 
 REF = "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -52,6 +57,8 @@ def main():
     with open(os.path.join(HERE, "reference_logs.json"), "w") as f:
         json.dump(doc, f, indent=1)
         f.write("\n")
+    for name in ("data_MN_K_512_128", "data_MN_K_512_128_sort"):
+        shutil.copyfile(os.path.join(REF, "ROCm", "data", name), os.path.join(HERE, name))
 
 
 if __name__ == "__main__":

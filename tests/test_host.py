@@ -252,3 +252,27 @@ def test_opts_struct_matches_header(bg):
     fields = re.findall(r"\b(?:int|unsigned)\s+([a-z_]+)\s*;", body)
     assert fields == [f for f, _ in bg._lib.vbg_opts._fields_]
     assert ctypes.sizeof(bg._lib.vbg_opts) == 4 * len(fields)
+
+
+def test_wang_policy_matches_oracle_and_ignores_input_order(bg, orc):
+    """The library's Wang policy (ROCm/tiling/gemm_wang.cuh:104-182 on the
+    sort.cpp order) against the oracle's restatement, and the same choice
+    for any order of the batch."""
+    M, N, K = orc.read_dataset(DATA)
+    cases = [(M[:B], N[:B], K[:B], th) for B in (8, 64, 256) for th in (40, 4000, 65536, 1 << 20)]
+    cases.append(([128] * 6, [128] * 6, [10, 50, 30, 70, 20, 60], 40))
+    rng = np.random.default_rng(3)
+    # distinct sort keys (K) so the sorted order, and hence the choice, cannot
+    # depend on the input order (ties keep input order, as in the reference)
+    mm = 32 * rng.integers(1, 9, 120); nn = 32 * rng.integers(1, 9, 120)
+    kk = rng.permutation(np.arange(8, 8 + 120))
+    cases += [(mm, nn, kk, th) for th in (4000, 65536, 400000)]
+    for i, (m, n, k, th) in enumerate(cases):
+        m, n, k = (np.asarray(x, np.int32) for x in (m, n, k))
+        p = bg.Plan(m, n, k, host_only=True, policy="wang", tlp_thres=th)
+        want = orc.wang_plan(m, n, k, th)
+        assert np.array_equal(p.tiles(), want), (len(m), th)
+        if len(set(k.tolist())) == len(k):
+            perm = rng.permutation(len(m))
+            q = bg.Plan(m[perm], n[perm], k[perm], host_only=True, policy="wang", tlp_thres=th)
+            assert np.array_equal(q.tiles(), p.tiles()[perm]), i

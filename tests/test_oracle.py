@@ -239,3 +239,33 @@ def test_fp16_scaled_matches_numpy():
     assert 2 ** 14 <= np.abs(x).max() * 2.0 ** e < 2 ** 15
     q = o.fp16_scaled(x, e)
     assert np.array_equal(q, (x * np.float32(2.0 ** e)).astype(np.float16).astype(np.float32))
+
+
+def test_wang_order_matches_reference_sorted_dataset(orc):
+    """sort.cpp's output for the first 256 dataset lines is shipped with the
+    reference (ROCm/data/data_MN_K_512_128_sort)."""
+    M, N, K = orc.read_dataset(os.path.join(GOLD, "data_MN_K_512_128"))
+    gold = np.loadtxt(os.path.join(GOLD, "data_MN_K_512_128_sort"), dtype=np.int64)
+    o = orc.wang_order(M[:256], N[:256], K[:256])
+    assert np.array_equal(np.stack([M[:256][o], N[:256][o], K[:256][o]], 1), gold)
+    # stable: equal keys keep their input order
+    assert all(o[i] < o[i + 1] for i in range(255)
+               if (K[:256][o[i]], M[:256][o[i]] % 64, N[:256][o[i]] % 64) ==
+               (K[:256][o[i + 1]], M[:256][o[i + 1]] % 64, N[:256][o[i + 1]] % 64))
+
+
+def test_wang_plan_invariants(orc):
+    M, N, K = orc.read_dataset(os.path.join(GOLD, "data_MN_K_512_128"))
+    for B in (8, 64, 256):
+        for thres in (2000, 65536, 1 << 20):
+            st = orc.wang_plan(M[:B], N[:B], K[:B], thres)
+            ts = np.array([16, 32, 64])[st]
+            assert ((M[:B] % ts == 0) & (N[:B] % ts == 0)).all()   # chosen tiles divide
+            assert st.max() <= 2 and st.min() >= 0
+    # hand-derived: six 128x128 matrices, thres 40.  No tile gives TLP < 40,
+    # so the global tile is 64 (index 2); the initial workgroup count is
+    # 6 * 4 = 24 < 40, so the demotion walks the sorted order (K descending:
+    # 70, 60, 50, 30, 20, 10) from its end: K=10 -> 32x32 (24 + 12 = 36),
+    # K=20 -> 32x32 (48 >= 40): stop.
+    st = orc.wang_plan([128] * 6, [128] * 6, [10, 50, 30, 70, 20, 60], 40)
+    assert st.tolist() == [1, 2, 2, 2, 1, 2]
