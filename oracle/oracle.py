@@ -298,3 +298,32 @@ def wang_plan(M, N, K, thres: int) -> np.ndarray:
         wg += cdiv(M[j], ts[st[j]]) * cdiv(N[j], ts[st[j]]) * 3
         st[j] -= 1
     return np.array(st, np.int32)
+
+
+LI_TILES = ((16, 16), (32, 32), (64, 64), (128, 64), (64, 128), (128, 128))
+
+
+def li_plan(M, N, thres: int) -> np.ndarray:
+    """Tile index per matrix over LI_TILES (the library's tile ids) of the Li
+    tiling strategy, CUDA/batching/gemm.cu:99-127: the first tile whose
+    TLP = sum (M/tm)(N/tn)*256 is below thres (the last if none); per
+    matrix that tile when tm <= M and tn <= N, else the last smaller tile
+    with tm == M and tn <= N, else 16x16."""
+    M = [int(x) for x in M]; N = [int(x) for x in N]
+    t = 0
+    while t < 6:
+        if sum((M[j] // LI_TILES[t][0]) * (N[j] // LI_TILES[t][1]) * 256 for j in range(len(M))) < thres:
+            break
+        t += 1
+    t = min(t, 5)
+    out = []
+    for m, n in zip(M, N):
+        s = 0
+        if LI_TILES[t][0] <= m and LI_TILES[t][1] <= n:
+            s = t
+        else:
+            for k in range(t):
+                if LI_TILES[k][0] == m and LI_TILES[k][1] <= n:
+                    s = k
+        out.append(s)
+    return np.array(out, np.int32)

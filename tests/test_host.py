@@ -276,3 +276,14 @@ def test_wang_policy_matches_oracle_and_ignores_input_order(bg, orc):
             perm = rng.permutation(len(m))
             q = bg.Plan(m[perm], n[perm], k[perm], host_only=True, policy="wang", tlp_thres=th)
             assert np.array_equal(q.tiles(), p.tiles()[perm]), i
+
+
+def test_li_policy_matches_oracle(bg, orc):
+    """The library's Li policy (CUDA/batching/gemm.cu:99-127) against the
+    oracle's restatement on dataset prefixes and a hand-derived case."""
+    M, N, K = orc.read_dataset(DATA)
+    cases = [(M[:B], N[:B], K[:B], th) for B in (8, 64, 256) for th in (1000, 65536, 1 << 20, 1 << 24)]
+    cases.append(([128, 64, 16, 32], [128, 128, 16, 256], [8, 8, 8, 8], 1000))
+    for m, n, k, th in cases:
+        p = bg.Plan(m, n, k, host_only=True, policy="li", tlp_thres=th)
+        assert np.array_equal(p.tiles(), orc.li_plan(m, n, th)), (len(m), th)

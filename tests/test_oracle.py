@@ -269,3 +269,13 @@ def test_wang_plan_invariants(orc):
     # K=20 -> 32x32 (48 >= 40): stop.
     st = orc.wang_plan([128] * 6, [128] * 6, [10, 50, 30, 70, 20, 60], 40)
     assert st.tolist() == [1, 2, 2, 2, 1, 2]
+
+
+def test_li_plan_hand_derived(orc):
+    """M = (128, 64, 16, 32), N = (128, 128, 16, 256), thres 1000: TLP is
+    33024 (16x16), 8192 (32x32), 1536 (64x64), 512 (128x64) -> global tile
+    128x64; 128x128 fits it; 64x128 takes 64x64 (tm == M); 16x16 takes 16x16;
+    32x256 takes 32x32."""
+    assert orc.li_plan([128, 64, 16, 32], [128, 128, 16, 256], 1000).tolist() == [3, 2, 0, 1]
+    # no tile below the threshold: the last (128x128) is the global tile
+    assert orc.li_plan([256], [256], 1).tolist() == [5]
