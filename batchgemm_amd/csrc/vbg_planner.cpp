@@ -347,36 +347,52 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
 
   // 1a. launch consolidation (AUTO).  Each tile shape in use is one more
   // persistent launch, and a launch costs a fixed start-up and drain time that
-  // the per-matrix cost model does not see: a shape whose matrices would take
-  // a small fraction of the dominant launch's time, and cost less than that
-  // plus the launch overhead when tiled with the dominant shape instead, joins
-  // the dominant launch.  Slot times are in the cost model's units (one
-  // 128x128x32 k-step of one workgroup slot).  Measured: bf16x3 on the
-  // unbalanced B=2048 batch runs in one launch instead of four, 216 -> 229
-  // TFLOP/s.
+  // the per-matrix cost model does not see.  A launch's modelled duration is
+  // max(total cost / slots, longest item) in the cost model's units (one
+  // 128x128x32 k-step of one workgroup slot).  The matrices of a shape join
+  // the dominant launch (retiled with its shape) when that lengthens it by
+  // less than their own launch's duration plus kLaunchFixed, provided their
+  // launch is short: a small fraction of the dominant one, or no longer than
+  // a launch's fixed cost (tiny batches, where every launch is latency).
+  // Measured: bf16x3 on the unbalanced B=2048 batch runs in one launch
+  // instead of four (220 -> 223 TFLOP/s).
   if (o.policy == VBG_POLICY_AUTO && o.consolidate >= 0 && fam != FAM_SEQ && batch > 0) {
     const double kLaunchFixed = 8.0, kAbsorbFrac = 0.05;
-    double slot_time[NTILES] = {0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < batch; ++b)
-      if (M[b] > 0 && N[b] > 0) slot_time[tile[b]] += matrix_cost(tile[b], sh[b]);
-    for (int t = 0; t < NTILES; ++t) slot_time[t] /= (double)cus * occ(fam, t);
-    const int D = (int)(std::max_element(slot_time, slot_time + NTILES) - slot_time);
+    double tot[NTILES] = {0, 0, 0, 0, 0, 0}, longest[NTILES] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < batch; ++b) {
+      if (M[b] == 0 || N[b] == 0) continue;
+      tot[tile[b]] += matrix_cost(tile[b], sh[b]);
+      longest[tile[b]] = std::max(longest[tile[b]], item_cost(tile[b], sh[b]));
+    }
+    auto slots = [&](int t) { return (double)cus * occ(fam, t); };
+    auto dur = [&](int t) { return std::max(tot[t] / slots(t), longest[t]); };
+    int D = 0;
+    for (int t = 1; t < NTILES; ++t)
+      if (dur(t) > dur(D)) D = t;
     std::vector<int> order;
     for (int t = 0; t < NTILES; ++t)
-      if (t != D && slot_time[t] > 0) order.push_back(t);
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return slot_time[x] < slot_time[y]; });
+      if (t != D && tot[t] > 0) order.push_back(t);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return dur(x) < dur(y); });
     if (tile_allowed(prec, D, o.tile_mask)) {
       for (int t : order) {
-        if (slot_time[t] > kAbsorbFrac * slot_time[D]) break;
-        double add = 0.0;
+        const bool short_launch = tot[t] / slots(t) <= kAbsorbFrac * (tot[D] / slots(D)) ||
+                                  dur(t) <= kLaunchFixed;
+        if (!short_launch) continue;
+        double add = 0.0, add_longest = 0.0;
         for (int b = 0; b < batch; ++b)
-          if (tile[b] == t && M[b] > 0 && N[b] > 0) add += matrix_cost(D, sh[b]);
-        add /= (double)cus * occ(fam, D);
-        if (add < slot_time[t] + kLaunchFixed) {
+          if (tile[b] == t && M[b] > 0 && N[b] > 0) {
+            add += matrix_cost(D, sh[b]);
+            add_longest = std::max(add_longest, item_cost(D, sh[b]));
+          }
+        const double before = dur(D);
+        const double after = std::max((tot[D] + add) / slots(D), std::max(longest[D], add_longest));
+        if (after - before < dur(t) + kLaunchFixed) {
           for (int b = 0; b < batch; ++b)
             if (tile[b] == t) tile[b] = D;
-          slot_time[D] += add;
-          slot_time[t] = 0.0;
+          tot[D] += add;
+          longest[D] = std::max(longest[D], add_longest);
+          tot[t] = 0.0;
+          longest[t] = 0.0;
         }
       }
     }

@@ -24,6 +24,9 @@ The default policy "reference" takes the reference planner's tile choice
 under wavefront_thres (gemm.cuh:141-201); TLP_thres is the threshold of the
 Wang and Li policies.  Layout is column-major (immaterial for constant data).
 Details go to stderr; stdout carries the reference's lines only.
+--per-matrix-library times the same protocol on one library GEMM per matrix
+(torch.addmm, i.e. rocBLAS / hipBLASLt), the reference's own comparison
+program ROCm/default/gemm.cuh.
 """
 from __future__ import annotations
 
@@ -50,6 +53,9 @@ def main(argv=None) -> int:
     ap.add_argument("--policy", default="reference", choices=["reference", "auto", "wang", "li", "fixed"])
     ap.add_argument("--prec", default="fp32")
     ap.add_argument("--verify", action="store_true")
+    ap.add_argument("--per-matrix-library", action="store_true",
+                    help="time one library GEMM per matrix instead (torch.addmm: rocBLAS / "
+                         "hipBLASLt), the reference's baseline program ROCm/default/gemm.cuh:64-85")
     args = ap.parse_args(argv)
 
     import torch
@@ -63,17 +69,28 @@ def main(argv=None) -> int:
         vb.a(i).fill_(i + 1)
         vb.b(i).fill_(i + 2)
         vb.c(i).fill_(i + 3)
-    plan = Plan(M, N, K, prec=args.prec, policy=args.policy, wavefront_thres=args.wavefront_thres,
-                tlp_thres=args.tlp_thres)
+    if args.per_matrix_library:
+        mats = [(vb.a(i), vb.b(i), vb.c(i)) for i in range(B)]
+
+        def run():  # C_i <- C_i + A_i B_i, one library call per matrix
+            for a, b, c in mats:
+                torch.addmm(c, a, b, beta=1.0, alpha=1.0, out=c)
+        plan = None
+    else:
+        plan = Plan(M, N, K, prec=args.prec, policy=args.policy,
+                    wavefront_thres=args.wavefront_thres, tlp_thres=args.tlp_thres)
+
+        def run():
+            plan.run(vb.dA, vb.dB, vb.dC, 1.0, 1.0)
     torch.cuda.synchronize()
 
-    plan.run(vb.dA, vb.dB, vb.dC, 1.0, 1.0)  # warm-up
+    run()  # warm-up
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record()
     for _ in range(N_RUNS):
-        plan.run(vb.dA, vb.dB, vb.dC, 1.0, 1.0)
+        run()
         torch.cuda.synchronize()
     stop.record()
     stop.synchronize()
@@ -82,12 +99,13 @@ def main(argv=None) -> int:
     print(f"time======{t:f}========N_RUNS====%d=====" % N_RUNS)
     print(f"gflops=={gflop:f}         {gflop / t:f}")
     sys.stdout.flush()
-    print(f"[driver] B={B} policy={args.policy} prec={args.prec} items={plan.n_items} "
-          f"launches={plan.n_launches}; 2*sum(MNK) rate {w.flops(M, N, K) / t / 1e9:.1f} GFLOP/s",
+    what = (f"per-matrix library, {B} launches" if plan is None else
+            f"policy={args.policy} prec={args.prec} items={plan.n_items} launches={plan.n_launches}")
+    print(f"[driver] B={B} {what}; 2*sum(MNK) rate {w.flops(M, N, K) / t / 1e9:.1f} GFLOP/s",
           file=sys.stderr)
 
     if args.verify:
-        seq = args.prec == "fp32_seq"
+        seq = args.prec == "fp32_seq" and plan is not None
         want = sequential_fp32(K, N_RUNS + 1) if seq else None
         bad = 0
         for i in range(B):
