@@ -86,10 +86,13 @@ void choose_auto(const std::vector<Shape>& sh, int prec, const vbg_opts& o, int 
     }
     tile[b] = best < 0 ? T64 : best;
   }
-  // Undersubscription repair.
+  // Undersubscription repair.  It stops at 64x64: on small batches the
+  // 32x32 / 16x16 kernels' per-item latency outweighs the extra parallelism
+  // (measured on the reference's dataset at B = 8 / 16: 64x64 items run the
+  // batch in 29 / 34 us, 32x32 items in 51 / 70 us).
   int64_t items = 0;
   for (int b = 0; b < B; ++b) items += num_tiles(tile[b], sh[b]);
-  const int smaller[NTILES] = {-1, T16, T32, T64, T64, T128x64};
+  const int smaller[NTILES] = {-1, -1, -1, T64, T64, T128x64};
   for (int iter = 0; iter < 4 * B && items < slots; ++iter) {
     int pick = -1;
     double longest = 0;
@@ -457,11 +460,29 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   // separate without edges): FP32 96 / 109 / 106, bf16x3 156 / 209 / 218,
   // FP16 206 / 192 / 202, FP8 206 / 192 / 202 -- the combined kernel loses
   // the 128x128 body's accumulator-in-AGPR allocation that FP32 and bf16x3
-  // run fastest with.  Default (0): mixed for FP16 / FP8; edge tiles where
-  // the launches are mixed and for the FP32 and BF16 families.
+  // run fastest with.  Default (0): mixed for FP16 / FP8 and for small
+  // batches; edge tiles where the launches are mixed and for the FP32 and
+  // BF16 families.
+  //   A batch whose launches would all be short (modelled total under
+  // kSmallBatch units: every launch is start-up latency) also runs mixed: one
+  // launch instead of one per shape.
   const bool split_scaled = prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3;
+  bool small_batch = false;
+  {
+    const double kSmallBatch = 32.0;
+    double tot[NTILES] = {0, 0, 0, 0, 0, 0}, longest[NTILES] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < batch; ++b) {
+      if (M[b] == 0 || N[b] == 0) continue;
+      tot[tile[b]] += matrix_cost(tile[b], sh[b]);
+      longest[tile[b]] = std::max(longest[tile[b]], item_cost(tile[b], sh[b]));
+    }
+    double total = 0.0;
+    for (int t = 0; t < NTILES; ++t)
+      if (tot[t] > 0) total += std::max(tot[t] / ((double)cus * occ(fam, t)), longest[t]);
+    small_batch = total < kSmallBatch;
+  }
   const bool mixed = (fam == FAM_FP32 || fam == FAM_SPLIT) &&
-                     (o.mixed_tiles > 0 || (o.mixed_tiles == 0 && split_scaled));
+                     (o.mixed_tiles > 0 || (o.mixed_tiles == 0 && (split_scaled || small_batch)));
   const bool edges = fam != FAM_SEQ && (o.edge_tiles > 0 ||
                                         (o.edge_tiles == 0 && (mixed || fam == FAM_FP32 || fam == FAM_BF16)));
   for (int b = 0; b < batch; ++b) {

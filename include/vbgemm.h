@@ -108,7 +108,8 @@ typedef struct vbg_opts {
   int mixed_tiles;      /* > 0: FP32 and split-precision plans run every 64x64 .. 128x128 tile
                            (interior tiles, edge strips, medium matrices) in one persistent
                            launch whose items carry their own shape; < 0: one launch per tile
-                           shape; 0 = default (mixed for FP16 / FP8 only) */
+                           shape; 0 = default (mixed for FP16 / FP8, and for batches whose
+                           launches would all be short) */
   int consolidate;      /* AUTO: >= 0 (default) moves the matrices of a tile shape whose launch
                            would be short into the dominant launch's shape when that is
                            modelled cheaper than a launch of their own; < 0 off */
