@@ -21,13 +21,6 @@ typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 typedef float v2f __attribute__((ext_vector_type(2)));
 
-// Round 4 floats to BF16 (nearest even) two at a time: the 2-wide form lowers
-// to one v_cvt_pk_bf16_f32 per pair.
-__device__ __forceinline__ v4bf bf16x4(v4f x) {
-  const v2bf a = __builtin_convertvector(v2f{x[0], x[1]}, v2bf);
-  const v2bf b = __builtin_convertvector(v2f{x[2], x[3]}, v2bf);
-  return __builtin_shufflevector(a, b, 0, 1, 2, 3);
-}
 // bf16x3 split of 4 floats: hi = bf16_rne(x), lo = bf16_rne(x - hi).  hi is
 // converted once, two values per v_cvt_pk_bf16_f32, and recovered as floats
 // from the packed word itself (low half: shift left 16; high half: mask), so

@@ -36,7 +36,8 @@ def test_dataset_shape_facts(orc):
     assert set(np.unique(M)) <= {16, 32, 64, 128, 256, 512}
     assert set(np.unique(N)) <= {16, 32, 64, 128, 256, 512}
     assert set(np.unique(K)) <= {16, 32, 64, 128}
-    # 2*sum(MNK) restated in BASELINE.md for B=8 and B=256
+    # 2*sum(MNK) of the first 8 and 256 dataset lines (the reference's count
+    # without its 2MN term)
     assert abs(orc.flops(M[:8], N[:8], K[:8]) / 1e9 - 0.051708) < 5e-7
     assert abs(orc.flops(M[:256], N[:256], K[:256]) / 1e9 - 0.738640) < 5e-7
 
