@@ -535,6 +535,8 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   struct TPart {
     Part p;
     int t;
+    double cost = 0.0;  // modelled cost of the whole part (all slices)
+    double item = 0.0;  // modelled cost of one of its items
   };
   std::vector<std::vector<TPart>> groups;
   std::vector<int> group_tile;
@@ -544,7 +546,11 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
       if (parts[t].empty()) continue;
       const bool into_mix = mixed && t >= T64;
       std::vector<TPart> g;
-      for (const Part& x : parts[t]) (into_mix ? mix : g).push_back({x, t});
+      for (const Part& x : parts[t]) {
+        const double item = slice_cost(t, x.b);
+        const double cost = (double)(x.tm1 - x.tm0) * (x.tn1 - x.tn0) * hp.desc[x.b].ksplit * item;
+        (into_mix ? mix : g).push_back({x, t, cost, item});
+      }
       if (!into_mix) {
         groups.push_back(std::move(g));
         group_tile.push_back(t);
@@ -557,28 +563,28 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   }
   for (size_t gi = 0; gi < groups.size(); ++gi) {
     std::vector<TPart>& ps = groups[gi];
-    auto pcost = [&](const TPart& x) {
-      return (double)(x.p.tm1 - x.p.tm0) * (x.p.tn1 - x.p.tn0) * hp.desc[x.p.b].ksplit * slice_cost(x.t, x.p.b);
-    };
     Launch L;
     L.tile = group_tile[gi];
     L.family = fam;
-    // longest parts first, ties by matrix index for determinism
-    std::stable_sort(ps.begin(), ps.end(), [&](const TPart& x, const TPart& y) { return pcost(x) > pcost(y); });
+    // longest parts first, ties by matrix index for determinism (costs are
+    // computed once per part, not per comparison)
+    std::stable_sort(ps.begin(), ps.end(), [](const TPart& x, const TPart& y) { return x.cost > y.cost; });
     std::vector<std::vector<TPart>> qm(hp.nq);
     std::vector<double> load(hp.nq, 0.0);
     for (const TPart& x : ps) {
       const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       qm[q].push_back(x);
-      load[q] += pcost(x);
-      L.cost += pcost(x);
+      load[q] += x.cost;
+      L.cost += x.cost;
     }
     L.q_begin.push_back(0);
+    size_t n_items = 0;
+    for (const TPart& x : ps)
+      n_items += (size_t)(x.p.tm1 - x.p.tm0) * (x.p.tn1 - x.p.tn0) * hp.desc[x.p.b].ksplit;
+    L.items.reserve(n_items);
     for (int q = 0; q < hp.nq; ++q) {
       // per-item cost decreasing; a part's tiles stay together
-      std::stable_sort(qm[q].begin(), qm[q].end(), [&](const TPart& x, const TPart& y) {
-        return slice_cost(x.t, x.p.b) > slice_cost(y.t, y.p.b);
-      });
+      std::stable_sort(qm[q].begin(), qm[q].end(), [](const TPart& x, const TPart& y) { return x.item > y.item; });
       // tiles in runs of `grp` tile rows, column by column: the ~64 items an
       // XCD runs at once form a grp x (64/grp) block of the matrix, touching
       // grp A panels and 64/grp B panels instead of 64/tn rows x tn columns
