@@ -64,6 +64,7 @@ class vbg_opts(ctypes.Structure):
         ("timing_slots", ctypes.c_int),
         ("mixed_tiles", ctypes.c_int),
         ("consolidate", ctypes.c_int),
+        ("pack_operands", ctypes.c_int),
     ]
 
 

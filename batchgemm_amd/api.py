@@ -138,7 +138,13 @@ class Plan:
         return out[:5 * n].reshape(n, 5)
 
     KERNEL_KINDS = ("scale pre-pass", "fp32 mfma", "fp32 sequential", "split precision", "bf16",
-                    "split-K reduction")
+                    "split-K reduction", "operand pack", "split precision (packed operands)")
+
+    @property
+    def packed_bytes(self) -> int:
+        """Bytes of the plan's packed-operand workspace (0: operands are
+        quantised on the way into LDS instead)."""
+        return self.query(8)
 
     def kernels(self) -> np.ndarray:
         """Kernels a run issues, in order, as rows (kind, tile id, first item,

@@ -34,6 +34,7 @@ int occupancy_cb(int family, int prec, int tile, bool ta, bool tb) {
     case vbg::FAM_FP32: return vbg_occupancy_fp32(tile, ta, tb);
     case vbg::FAM_SEQ: return vbg_occupancy_seq(ta, tb);
     case vbg::FAM_SPLIT: return vbg_occupancy_split(prec, tile, ta, tb);
+    case vbg::FAM_SPLIT_PK: return vbg_occupancy_split_pk(prec, tile);
     case vbg::FAM_BF16: return vbg_occupancy_bf16(prec, tile, ta, tb);
     default: return 1;
   }
@@ -54,6 +55,9 @@ struct vbg_plan {
   vbg_chunk* d_red = nullptr;     // split-K reduction chunks
   int64_t* d_ws_off = nullptr;    // per-matrix workspace offsets (floats)
   float* d_ws = nullptr;          // split-K partial panels (own allocation)
+  char* d_pk = nullptr;           // packed operands (own allocation)
+  int64_t* d_pk_off = nullptr;    // [2 * batch] byte offsets into d_pk
+  vbg_pack_job* d_jobs = nullptr; // pack pre-pass jobs
   struct DevLaunch {
     vbg_item* items;
     int* q_begin;
@@ -72,9 +76,13 @@ bool has_prepass(const vbg::HostPlan& hp) {
   return hp.prec == VBG_PREC_FP16_SCALED || hp.prec == VBG_PREC_FP8_E4M3;
 }
 
-// Kernels a run issues, in order: [scale pre-pass], one per launch, [split-K reduction].
+bool has_pack(const vbg::HostPlan& hp) { return !hp.pack_jobs.empty(); }
+
+// Kernels a run issues, in order: [scale pre-pass], [operand pack], one per
+// launch, [split-K reduction].
 int kernel_rows(const vbg::HostPlan& hp) {
-  return (has_prepass(hp) ? 1 : 0) + (int)hp.launches.size() + (hp.red_chunks.empty() ? 0 : 1);
+  return (has_prepass(hp) ? 1 : 0) + (has_pack(hp) ? 1 : 0) + (int)hp.launches.size() +
+         (hp.red_chunks.empty() ? 0 : 1);
 }
 
 int family_kind(int fam) {
@@ -83,6 +91,7 @@ int family_kind(int fam) {
     case vbg::FAM_SEQ: return 2;
     case vbg::FAM_SPLIT: return 3;
     case vbg::FAM_BF16: return 4;
+    case vbg::FAM_SPLIT_PK: return 7;
     default: return -1;
   }
 }
@@ -188,6 +197,10 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     off = align256(off + sizeof(vbg_chunk) * hp.red_chunks.size());
     const size_t off_wsoff = off;
     off = align256(off + sizeof(int64_t) * std::max(1, hp.batch));
+    const size_t off_pkoff = off;
+    off = align256(off + sizeof(int64_t) * hp.pk_off.size());
+    const size_t off_jobs = off;
+    off = align256(off + sizeof(vbg_pack_job) * hp.pack_jobs.size());
     std::vector<size_t> oi, oq, oc;
     for (const auto& L : hp.launches) {
       oi.push_back(off);
@@ -204,6 +217,10 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     if (!hp.red_chunks.empty())
       std::memcpy(host.data() + off_red, hp.red_chunks.data(), sizeof(vbg_chunk) * hp.red_chunks.size());
     if (hp.batch) std::memcpy(host.data() + off_wsoff, hp.ws_off.data(), sizeof(int64_t) * hp.batch);
+    if (!hp.pk_off.empty())
+      std::memcpy(host.data() + off_pkoff, hp.pk_off.data(), sizeof(int64_t) * hp.pk_off.size());
+    if (!hp.pack_jobs.empty())
+      std::memcpy(host.data() + off_jobs, hp.pack_jobs.data(), sizeof(vbg_pack_job) * hp.pack_jobs.size());
     for (size_t l = 0; l < hp.launches.size(); ++l) {
       const auto& L = hp.launches[l];
       std::memcpy(host.data() + oi[l], L.items.data(), sizeof(vbg_item) * L.items.size());
@@ -227,12 +244,23 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
     p->d_chunks = (vbg_chunk*)(base + off_chunks);
     p->d_red = (vbg_chunk*)(base + off_red);
     p->d_ws_off = (int64_t*)(base + off_wsoff);
+    p->d_pk_off = (int64_t*)(base + off_pkoff);
+    p->d_jobs = (vbg_pack_job*)(base + off_jobs);
     if (hp.ws_floats > 0) {
       e = hipMalloc(&p->d_ws, sizeof(float) * hp.ws_floats);
       if (e != hipSuccess) {
         (void)hipFree(p->dbuf);
         delete p;
         return hip_fail(e, "hipMalloc(split-K workspace)");
+      }
+    }
+    if (hp.pk_bytes > 0) {
+      e = hipMalloc(&p->d_pk, (size_t)hp.pk_bytes);
+      if (e != hipSuccess) {
+        int rc2 = hip_fail(e, "hipMalloc(packed operands)");
+        p->d_pk = nullptr;
+        vbg_plan_destroy(p);
+        return rc2;
       }
     }
     for (size_t l = 0; l < hp.launches.size(); ++l)
@@ -265,6 +293,10 @@ int vbg_plan_destroy(vbg_plan* p) {
   if (p->d_ws) {
     hipError_t e = hipFree(p->d_ws);
     if (e != hipSuccess) rc = hip_fail(e, "hipFree(workspace)");
+  }
+  if (p->d_pk) {
+    hipError_t e = hipFree(p->d_pk);
+    if (e != hipSuccess) rc = hip_fail(e, "hipFree(packed operands)");
   }
   delete p;
   return rc;
@@ -308,6 +340,12 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
     if (r) return fail(VBG_ERR_HIP, std::string("scale pre-pass launch: ") + hipGetErrorString(hipGetLastError()));
     if (int rc = mark()) return rc;
   }
+  if (has_pack(hp)) {
+    int r = vbg_launch_pack(hp.prec, hp.ta, hp.tb, p->d_jobs, (int)hp.pack_jobs.size(), p->d_desc, dA, dB,
+                            p->d_scales, p->d_pk, p->d_pk_off, s);
+    if (r) return fail(VBG_ERR_HIP, std::string("operand pack launch: ") + hipGetErrorString(hipGetLastError()));
+    if (int rc = mark()) return rc;
+  }
   for (size_t l = 0; l < hp.launches.size(); ++l) {
     const vbg::Launch& L = hp.launches[l];
     vbg_args a;
@@ -321,6 +359,8 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
     a.ws = p->d_ws;
     a.ws_off = p->d_ws_off;
     a.scales = p->d_scales;
+    a.pk = L.family == vbg::FAM_SPLIT_PK ? p->d_pk : nullptr;
+    a.pk_off = p->d_pk_off;
     a.alpha = alpha;
     a.beta = beta;
     a.nq = hp.nq;
@@ -332,6 +372,7 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
       case vbg::FAM_FP32: r = vbg_launch_fp32(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
       case vbg::FAM_SEQ: r = vbg_launch_seq(hp.ta, hp.tb, &a, L.grid, s); break;
       case vbg::FAM_SPLIT: r = vbg_launch_split(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
+      case vbg::FAM_SPLIT_PK: r = vbg_launch_split_pk(L.tile, &a, L.grid, s); break;
       case vbg::FAM_BF16: r = vbg_launch_bf16(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
       default: r = -1;
     }
@@ -448,6 +489,7 @@ int vbg_plan_query(const vbg_plan* p, int what, int* out, int out_len) {
         out[w++] = grid;
       };
       if (has_prepass(hp)) put(0, -1, 0, 0, (int)hp.chunks.size());
+      if (has_pack(hp)) put(6, -1, 0, 0, (int)hp.pack_jobs.size());
       for (const auto& L : hp.launches) {
         put(family_kind(L.family), L.tile, first, (int)L.items.size(), L.grid);
         first += (int)L.items.size();
@@ -455,6 +497,7 @@ int vbg_plan_query(const vbg_plan* p, int what, int* out, int out_len) {
       if (!hp.red_chunks.empty()) put(5, -1, 0, 0, (int)hp.red_chunks.size());
       return w;
     }
+    case 8: return (int)std::min<int64_t>(hp.pk_bytes, 0x7fffffff);
     default: return fail(VBG_ERR_INVALID_ARG, "unknown query");
   }
 }

@@ -12,6 +12,8 @@
 // (ROCm/framework/kernel_128_new.h, kernel_256_new.h) and its dispatcher
 // (ROCm/framework/kernel.h:9-42): a work item names the matrix and tile, so no
 // workgroup is launched for an empty bounding-box slot.
+#include <type_traits>
+
 #include "vbg_tiles.h"
 
 namespace {
@@ -21,7 +23,9 @@ namespace {
 // Any assignment of actual k to (slot, step) works as long as A and B agree;
 // lane half h takes k in [h*BK/2, (h+1)*BK/2), so each lane reads its k values
 // as contiguous float4s from the [mn][k] LDS layout.
-template <int BM, int BN, int BK, bool TA, bool TB>
+// CPRE: the last k-tile loads the epilogue's first chunk of C (off inside the
+// mixed kernel, where four bodies share one register budget).
+template <int BM, int BN, int BK, bool TA, bool TB, bool CPRE = true>
 struct Fp32MfmaTile {
   static constexpr int P = BK + 4;  // LDS pitch (floats); +4 staggers rows across banks
   static constexpr int STAGE = (BM + BN) * P;
@@ -71,13 +75,28 @@ struct Fp32MfmaTile {
     }
     __syncthreads();
 
-    for (int kt = 0; kt < nk; ++kt) {
+    // 32x32x2 f32 accumulator layout: register 4g+e of lane l holds
+    // row 8g + 4*(l/32) + e, column l%32 of the 32x32 block.
+    constexpr int NSEG = TM * TN * 4;
+    const OutTile o = vbg_out(a, d, it, 1.0f);
+    auto seg = [&](int sg, int& row, int& col, v4f& v) {
+      const int r = sg / (TN * 4), s = (sg / 4) % TN, g = sg % 4;
+      row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
+      col = n0 + wc * WTN + s * 32 + l32;
+      v = v4f{acc[r][s][4 * g], acc[r][s][4 * g + 1], acc[r][s][4 * g + 2], acc[r][s][4 * g + 3]};
+    };
+    // one k-tile; the last one (peeled) loads the epilogue's first chunk of
+    // C in place of a next tile, so that round trip overlaps its MFMAs
+    v4f cpre[EpiChunk<NSEG>::CH];
+    auto ktile = [&](int kt, auto last, auto pre) {
+      constexpr bool LAST = decltype(last)::value;
       const int cur = kt & 1;
-      const bool more = kt + 1 < nk;
-      if (more) {
+      if constexpr (!LAST) {
         const int k0 = kb + (kt + 1) * BK;
         la.fetch(ra, k0, ke);
         lb.fetch(rb, k0, ke);
+      } else if constexpr (decltype(pre)::value) {
+        vbg_c_load<NSEG>(o, d.M, d.N, seg, 0, cpre);
       }
       const float* As = lds + cur * STAGE;
       const float* Bs = As + BM * P;
@@ -98,23 +117,25 @@ struct Fp32MfmaTile {
             for (int s = 0; s < TN; ++s)
               acc[r][s] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[r][e], bv[s][e], acc[r][s], 0, 0, 0);
       }
-      if (more) {
+      if constexpr (!LAST) {
         float* nxt = lds + (cur ^ 1) * STAGE;
         LA::store(ra, nxt);
         LB::store(rb, nxt + BM * P);
       }
       __syncthreads();
+    };
+    for (int kt = 0; kt + 1 < nk; ++kt) ktile(kt, std::false_type{}, std::false_type{});
+    if constexpr (CPRE) {
+      if (nk > 0) {
+        ktile(nk - 1, std::true_type{}, std::true_type{});
+        vbg_epilogue<NSEG>(o, d.M, d.N, seg, cpre);
+      } else {
+        vbg_epilogue<NSEG>(o, d.M, d.N, seg);
+      }
+    } else {
+      if (nk > 0) ktile(nk - 1, std::true_type{}, std::false_type{});
+      vbg_epilogue<NSEG>(o, d.M, d.N, seg);
     }
-
-    // 32x32x2 f32 accumulator layout: register 4g+e of lane l holds
-    // row 8g + 4*(l/32) + e, column l%32 of the 32x32 block.
-    const OutTile o = vbg_out(a, d, it, 1.0f);
-    vbg_epilogue<TM * TN * 4>(o, d.M, d.N, [&](int sg, int& row, int& col, v4f& v) {
-      const int r = sg / (TN * 4), s = (sg / 4) % TN, g = sg % 4;
-      row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
-      col = n0 + wc * WTN + s * 32 + l32;
-      v = v4f{acc[r][s][4 * g], acc[r][s][4 * g + 1], acc[r][s][4 * g + 2], acc[r][s][4 * g + 3]};
-    });
   }
 };
 
@@ -313,10 +334,10 @@ vbg_gemm_kernel(vbg_args a) {
 // the largest; registers are those of the largest body.
 template <bool TA, bool TB>
 struct Fp32Mixed {
-  using Sq128 = Fp32MfmaTile<128, 128, 32, TA, TB>;
-  using Tall = Fp32MfmaTile<128, 64, 32, TA, TB>;
-  using Wide = Fp32MfmaTile<64, 128, 32, TA, TB>;
-  using Sq64 = Fp32MfmaTile<64, 64, 32, TA, TB>;
+  using Sq128 = Fp32MfmaTile<128, 128, 32, TA, TB, false>;
+  using Tall = Fp32MfmaTile<128, 64, 32, TA, TB, false>;
+  using Wide = Fp32MfmaTile<64, 128, 32, TA, TB, false>;
+  using Sq64 = Fp32MfmaTile<64, 64, 32, TA, TB, false>;
   static constexpr int LDS_FLOATS = Sq128::LDS_FLOATS;
   static constexpr int MIN_WAVES = 2;  // two workgroups per CU, as the 128x128 body alone
   static_assert(Sq128::LDS_FLOATS >= Tall::LDS_FLOATS && Sq128::LDS_FLOATS >= Sq64::LDS_FLOATS,

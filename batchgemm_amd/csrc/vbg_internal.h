@@ -27,6 +27,19 @@ struct vbg_chunk {
   int64_t begin, end;
 };
 
+// One job of the operand pack pre-pass (split precisions with packed
+// operands): rows [row0, row0 + 64) x k [k0, k0 + VBG_PACK_K) of operand `op`
+// (0 = A, 1 = B) of matrix `mat`, written quantised to the packed workspace.
+struct vbg_pack_job {
+  int mat;
+  int op;
+  int row0, k0;
+};
+// Packed operands hold K rounded up to this many elements per row (whole
+// k-tiles for every split tile depth, zeros past K).
+#define VBG_PACK_KALIGN 64
+#define VBG_PACK_K 256
+
 // Arguments of every persistent GEMM kernel (passed by value).
 struct vbg_args {
   const vbg_mat_desc* desc;
@@ -39,6 +52,8 @@ struct vbg_args {
   float* ws;              // split-K partial sums (ws_off per item slice), or null
   const int64_t* ws_off;  // per-matrix offset into ws (floats), or null
   const float* scales;    // per-matrix quantisation scales [2*batch] (FP16 / FP8), or null
+  const char* pk;         // packed operands (split precisions), or null: operands from A / B
+  const int64_t* pk_off;  // [2*batch] byte offsets of packed A and B in pk
   float alpha, beta;
   int nq;
   int trans_a, trans_b;   // 0 = N, 1 = T

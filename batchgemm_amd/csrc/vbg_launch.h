@@ -12,6 +12,14 @@ int vbg_occupancy_seq(int ta, int tb);
 // Split-precision family (bf16x3, FP16 scaled, FP8 e4m3); a->prec selects.
 int vbg_launch_split(int tile, int ta, int tb, const vbg_args* a, int grid, void* stream);
 int vbg_occupancy_split(int prec, int tile, int ta, int tb);
+// The same kernels on packed operands (a->pk, a->pk_off), and the operand
+// pack pre-pass that writes them (one workgroup per job; FP16 / FP8 read the
+// scales of the amax pre-pass).
+int vbg_launch_split_pk(int tile, const vbg_args* a, int grid, void* stream);
+int vbg_occupancy_split_pk(int prec, int tile);
+int vbg_launch_pack(int prec, int ta, int tb, const vbg_pack_job* jobs, int n_jobs,
+                    const vbg_mat_desc* desc, const float* const* A, const float* const* B,
+                    const float* scales, char* pk, const int64_t* pk_off, void* stream);
 // Per-matrix amax pre-pass and scales (FP16 / FP8): zero amax, amax over the
 // chunks, then scales[4b..4b+2] = {sA, sB, 1/(sA*sB)}.
 int vbg_launch_scales(int prec, int batch, int ta, int tb, const vbg_mat_desc* desc,

@@ -288,6 +288,19 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
       err = "BF16 operands need lda and ldb multiples of 8 (matrix " + std::to_string(b) + ")";
       return VBG_ERR_NOT_SUPPORTED;
     }
+    // The kernels address every operand with 32-bit byte offsets from its
+    // base (buffer resources), so each stored matrix must span < 2 GiB.
+    {
+      const int64_t eb = (prec == VBG_PREC_BF16 || prec == VBG_PREC_BF16_OUT_BF16) ? 2 : 4;
+      const int64_t lim = (int64_t)1 << 31;
+      const int64_t span_a = (int64_t)lda[b] * (ta ? M[b] : K[b]) * eb;
+      const int64_t span_b = (int64_t)ldb[b] * (tb ? K[b] : N[b]) * eb;
+      const int64_t span_c = (int64_t)ldc[b] * N[b] * 4;
+      if (span_a >= lim || span_b >= lim || span_c >= lim) {
+        err = "matrix " + std::to_string(b) + " spans 2 GiB or more (32-bit offsets)";
+        return VBG_ERR_NOT_SUPPORTED;
+      }
+    }
     sh[b] = {M[b], N[b], K[b]};
     hp.desc[b] = {M[b], N[b], K[b], lda[b], ldb[b], ldc[b], 0, 1};
   }
@@ -343,9 +356,34 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   if (is_split(prec))
     for (int b = 0; b < batch; ++b) tile[b] = std::max(tile[b], (int)T64);
   hp.tile_of.assign(batch, -1);
+  // Packed operands for bf16x3 / FP16 / FP8: one pre-pass quantises every
+  // operand into k-contiguous rows (4 / 2 / 1 bytes per element), which the
+  // GEMM kernels then stream with no conversion on the HBM -> LDS path.  The
+  // pass costs a read and a write of the operands (about 4.3 TB/s), so by
+  // default an FP16 / FP8 batch packs when its FP32 arithmetic intensity,
+  // 2*sum(MNK) / 4*sum(MK + KN) flop/B, reaches kPackMinAI.  Measured, ms
+  // (unpacked / packed incl. the pass): unbalanced B=2048 (AI ~156) FP16
+  // 3.85 / 3.57, FP8 3.86 / 3.18; log-uniform 8..1024, B=4096 (AI ~14) FP16
+  // 1.25 / 1.55, FP8 1.26 / 1.43.  bf16x3 stays unpacked by default: its
+  // GEMM kernel runs no faster on packed operands (3.80 / 3.95 ms on the
+  // unbalanced batch; the conversion is not what bounds it), so the pass
+  // (1.26 ms) is pure cost.
+  const bool packable = prec == VBG_PREC_BF16X3 || prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3;
+  bool pack = packable && o.pack_operands > 0;
+  if (packable && o.pack_operands == 0 && prec != VBG_PREC_BF16X3) {
+    const double kPackMinAI = 64.0;
+    double flops = 0.0, bytes = 0.0;
+    for (int b = 0; b < batch; ++b) {
+      if (M[b] == 0 || N[b] == 0 || K[b] == 0) continue;
+      flops += 2.0 * M[b] * N[b] * K[b];
+      bytes += 4.0 * ((double)M[b] * K[b] + (double)K[b] * N[b]);
+    }
+    pack = bytes > 0.0 && flops / bytes >= kPackMinAI;
+  }
   const int fam = prec == VBG_PREC_FP32_SEQ ? FAM_SEQ
                   : prec == VBG_PREC_FP32   ? FAM_FP32
                   : (prec == VBG_PREC_BF16 || prec == VBG_PREC_BF16_OUT_BF16) ? FAM_BF16
+                  : pack                    ? FAM_SPLIT_PK
                                             : FAM_SPLIT;
 
   // 1a. launch consolidation (AUTO).  Each tile shape in use is one more
@@ -487,7 +525,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   // batching workload (log-uniform sizes, about 25% outside): mixed is
   // +11% for FP32 and +13% for bf16x3; on the unbalanced B=2048 batch (about
   // 4% outside) separate launches stay faster.
-  const bool mix_family = fam == FAM_FP32 || fam == FAM_SPLIT;
+  const bool mix_family = fam == FAM_FP32 || fam == FAM_SPLIT || fam == FAM_SPLIT_PK;
   auto build_parts = [&](bool edges) {
     for (auto& v : parts) v.clear();
     for (int b = 0; b < batch; ++b) {
@@ -630,6 +668,32 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
         }
       }
     }
+  }
+  if (pack) {
+    // Packed A (M rows) then packed B (N rows) per matrix, each of Kp = K
+    // rounded up to VBG_PACK_KALIGN elements per row and 256-byte aligned;
+    // one pack job per 64 rows x VBG_PACK_K k.  Matrices with K = 0 have no
+    // packed operands (their items only scale C).
+    const int ebp = prec == VBG_PREC_BF16X3 ? 4 : prec == VBG_PREC_FP16_SCALED ? 2 : 1;
+    hp.pk_off.assign(2 * (size_t)batch, 0);
+    int64_t off = 0;
+    for (int b = 0; b < batch; ++b) {
+      if (M[b] == 0 || N[b] == 0 || K[b] == 0) continue;
+      const int64_t Kp = (int64_t)cdiv(K[b], VBG_PACK_KALIGN) * VBG_PACK_KALIGN;
+      for (int op = 0; op < 2; ++op) {
+        const int rows = op == 0 ? M[b] : N[b];
+        const int64_t bytes = (int64_t)rows * Kp * ebp;
+        if (bytes >= ((int64_t)1 << 31)) {
+          err = "packed operand of matrix " + std::to_string(b) + " exceeds 2 GiB";
+          return VBG_ERR_NOT_SUPPORTED;
+        }
+        hp.pk_off[2 * (size_t)b + op] = off;
+        off += (bytes + 255) & ~(int64_t)255;
+        for (int r0 = 0; r0 < rows; r0 += 64)
+          for (int k0 = 0; k0 < Kp; k0 += VBG_PACK_K) hp.pack_jobs.push_back({b, op, r0, k0});
+      }
+    }
+    hp.pk_bytes = off;
   }
   return VBG_OK;
 }

@@ -15,7 +15,7 @@ namespace vbg {
 
 // Kernel family a tile id runs on for a precision (one persistent launch per
 // family used by a plan).
-enum Family { FAM_FP32 = 0, FAM_SEQ = 1, FAM_SPLIT = 2, FAM_BF16 = 3 };
+enum Family { FAM_FP32 = 0, FAM_SEQ = 1, FAM_SPLIT = 2, FAM_BF16 = 3, FAM_SPLIT_PK = 4 };
 
 struct Launch {
   int tile = 0;                 // tile id (T16 .. T128), or T_MIXED (tile per item)
@@ -38,6 +38,11 @@ struct HostPlan {
   std::vector<int64_t> ws_off;   // per matrix (floats), -1 if not split
   std::vector<vbg_chunk> chunks; // amax pre-pass slices (FP16 / FP8)
   std::vector<vbg_chunk> red_chunks;  // split-K reduction slices
+  // Packed operands (FAM_SPLIT_PK): byte offsets of packed A and B per matrix
+  // (2 * batch), the workspace size, and the pack pre-pass jobs.
+  std::vector<int64_t> pk_off;
+  int64_t pk_bytes = 0;
+  std::vector<vbg_pack_job> pack_jobs;
 };
 
 // Fills `hp` from the shapes and options.  `occupancy(tile)` gives the

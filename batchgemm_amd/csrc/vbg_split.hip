@@ -9,6 +9,8 @@
 //            result scaled by 1/(sA*sB)
 // No counterpart in the reference (its kernels are FP32 only); the tiling,
 // work map and persistent driver are shared with the FP32 family.
+#include <type_traits>
+
 #include "vbg_launch.h"
 #include "vbg_tiles.h"
 #include "../../include/vbgemm.h"
@@ -37,6 +39,7 @@ __device__ __forceinline__ void bf16x3_split4(v4f x, v4bf& hi, v4bf& lo) {
   lo = __builtin_shufflevector(__builtin_convertvector(r01, v2bf), __builtin_convertvector(r23, v2bf),
                                0, 1, 2, 3);
 }
+typedef unsigned v2u32_t __attribute__((ext_vector_type(2)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef _Float16 v4h __attribute__((ext_vector_type(4)));
 
@@ -76,6 +79,10 @@ struct Mode;
 #ifndef VBG_BF16X3_STAGES
 #define VBG_BF16X3_STAGES 1
 #endif
+// Register stages on packed operands (no conversion between load and LDS).
+#ifndef VBG_PK_STAGES
+#define VBG_PK_STAGES 2
+#endif
 template <>
 struct Mode<MODE_BF16X3> {
   static constexpr int EB = 2, PLANES = 2, STAGES = VBG_BF16X3_STAGES, WAVES = VBG_BF16X3_WAVES,
@@ -92,14 +99,64 @@ struct Mode<MODE_FP8> {
                        BK = VBG_FP16_BK;
 };
 
+// Loader of a packed operand: ROWS rows of Kp = K rounded up to
+// VBG_PACK_KALIGN elements (k contiguous, EB bytes each, zero past K), PLANES
+// planes of MN*Kp*EB bytes one after the other.  A k-tile row is BK*EB bytes
+// = UPR 16-byte units; consecutive threads take consecutive units of a row.
+// Rows past MN read row MN-1 (their results are never stored).
+template <int ROWS, int BK, int EB, int PLANES>
+struct PkLoader {
+  static constexpr bool PACKED = true;
+  static constexpr int UPR = BK * EB / 16;
+  static constexpr int PER_PLANE = ROWS * UPR;
+  static constexpr int UNITS = PER_PLANE * PLANES;
+  static constexpr int NV = (UNITS + VBG_THREADS - 1) / VBG_THREADS;
+  static constexpr bool FULL = UNITS % VBG_THREADS == 0;
+  static_assert(UPR >= 1 && (FULL || NV == 1), "partial use of the block only for one unit per thread");
+  typedef v4u32_t Regs[NV];
+  __amdgpu_buffer_rsrc_t rsrc;
+  int voff[NV];
+
+  __device__ __forceinline__ static bool active(int v) {
+    return FULL || (int)threadIdx.x + VBG_THREADS * v < UNITS;
+  }
+  __device__ __forceinline__ static void coords(int v, int& plane, int& row, int& u) {
+    const int idx = threadIdx.x + VBG_THREADS * v;
+    plane = idx / PER_PLANE;
+    const int r = idx % PER_PLANE;
+    row = r / UPR;
+    u = r % UPR;
+  }
+  __device__ __forceinline__ void init(const char* base, int Kp, int mn0, int MN) {
+    const uint32_t plane_bytes = (uint32_t)MN * (uint32_t)Kp * EB;
+    rsrc = vbg_rsrc(base, plane_bytes * PLANES);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      int plane, row, u;
+      coords(v, plane, row, u);
+      voff[v] = (int)(plane * plane_bytes) + (min(mn0 + row, MN - 1) * Kp * EB) + 16 * u;
+    }
+  }
+  // k-tile [k0, k0 + BK) (inside the packed rows: Kp is a multiple of BK).
+  __device__ __forceinline__ void fetch(Regs& r, int k0, int) const {
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      if (active(v)) r[v] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff[v], k0 * EB, 0);
+  }
+};
+
 // LDS layout per operand plane: [mn][k] with k contiguous, rows of BK*EB bytes
 // split into NU = BK/8 units of 8 k (16 B for BF16/FP16, 8 B for FP8: one MFMA
 // operand read); unit u of row r sits at unit u ^ ((r / RPB) % NU), RPB =
 // 256 / row bytes (at least 1), so the rows an MFMA operand read touches hit
 // distinct LDS banks without row padding.  Lane half h of an MFMA wave owns k
 // in [h*BK/2, (h+1)*BK/2); step s of the 32x32x16 MFMA takes 8 of them,
-// k = h*BK/2 + 8s + [0, 8).
-template <int BM, int BN, int MODE, bool TA, bool TB>
+// k = h*BK/2 + 8s + [0, 8).  With CPRE, the one-register-stage loop loads
+// the epilogue's first chunk of C during the last k-tile, into registers the
+// next tile's prefetch would otherwise hold.
+//   With PK the operands come already quantised from the packed workspace
+// (the operand pack pre-pass); TA / TB are then unused.
+template <int BM, int BN, int MODE, bool TA, bool TB, bool CPRE = true, bool PK = false>
 struct SplitTile {
   static constexpr int BK = Mode<MODE>::BK;
   static_assert(BK == 32 || BK == 64, "k-tile depth 32 or 64");
@@ -116,18 +173,35 @@ struct SplitTile {
   static constexpr int PLANE_A = BM * PITCH, PLANE_B = BN * PITCH;
   static constexpr int STAGE = PLANES * (PLANE_A + PLANE_B);  // bytes
   static constexpr int LDS_STAGES = Mode<MODE>::LDS_STAGES;
-  static_assert(LDS_STAGES == 2 || Mode<MODE>::STAGES == 1, "two register stages need two LDS buffers");
   static constexpr int LDS_FLOATS = LDS_STAGES * STAGE / 4;
   static constexpr int WTM = BM / 2, WTN = BN / 2;
   static constexpr int TM = WTM / 32, TN = WTN / 32;
   static_assert(TM >= 1 && TN >= 1, "split tile needs >= 64x64");
-  using LA = TileLoader<BM, BK, 4, TA>;  // P unused: stores are done here
-  using LB = TileLoader<BN, BK, 4, !TB>;
+  using LA = std::conditional_t<PK, PkLoader<BM, BK, EB, PLANES>, TileLoader<BM, BK, 4, TA>>;
+  using LB = std::conditional_t<PK, PkLoader<BN, BK, EB, PLANES>, TileLoader<BN, BK, 4, !TB>>;
+  static constexpr int STAGES = PK ? VBG_PK_STAGES : Mode<MODE>::STAGES;
+  static_assert(LDS_STAGES == 2 || STAGES == 1, "two register stages need two LDS buffers");
 
   // Quantise 4 consecutive k of one row and write them to the plane(s).
   template <class L>
   __device__ __forceinline__ static void store(const typename L::Regs& rr, char* plane0,
                                                int plane_bytes, float s) {
+    if constexpr (PK) {
+      (void)s;
+#pragma unroll
+      for (int v = 0; v < L::NV; ++v) {
+        if (!L::active(v)) continue;
+        int plane, row, u;
+        L::coords(v, plane, row, u);
+        char* dst = plane0 + plane * plane_bytes;
+        if constexpr (UNIT == 16) {
+          *(v4u32_t*)(dst + swz(row, 16 * u)) = rr[v];
+        } else {  // 8-byte units (FP8): the two halves land in different units
+          *(v2u32_t*)(dst + swz(row, 16 * u)) = v2u32_t{rr[v][0], rr[v][1]};
+          *(v2u32_t*)(dst + swz(row, 16 * u + 8)) = v2u32_t{rr[v][2], rr[v][3]};
+        }
+      }
+    } else {
 #pragma unroll
     for (int v = 0; v < L::NV; ++v) {
       int row, kq;
@@ -149,13 +223,14 @@ struct SplitTile {
         *(int*)dst = p;
       }
     }
+    }
   }
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds_f) {
     char* lds = (char*)lds_f;
     const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
-    const float* __restrict__ A = vbg_uptr(a.A[it.mat]);
-    const float* __restrict__ B = vbg_uptr(a.B[it.mat]);
+    const float* __restrict__ A = PK ? nullptr : vbg_uptr(a.A[it.mat]);
+    const float* __restrict__ B = PK ? nullptr : vbg_uptr(a.B[it.mat]);
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;
@@ -184,8 +259,14 @@ struct SplitTile {
     // buffer between the MFMA steps, and the loads of tile kt+2 are in flight.
     LA la;
     LB lb;
-    la.init(A, d.lda, m0, d.M, d.K);
-    lb.init(B, d.ldb, n0, d.N, d.K);
+    if constexpr (PK) {
+      const int Kp = (d.K + VBG_PACK_KALIGN - 1) & ~(VBG_PACK_KALIGN - 1);
+      la.init(a.pk + a.pk_off[2 * it.mat], Kp, m0, d.M);
+      lb.init(a.pk + a.pk_off[2 * it.mat + 1], Kp, n0, d.N);
+    } else {
+      la.init(A, d.lda, m0, d.M, d.K);
+      lb.init(B, d.ldb, n0, d.N, d.K);
+    }
     typename LA::Regs la0, la1;
     typename LB::Regs lb0, lb1;
     auto load = [&](typename LA::Regs& ra, typename LB::Regs& rb, int kt) {
@@ -269,7 +350,15 @@ struct SplitTile {
       });
       __syncthreads();
     };
-    if constexpr (Mode<MODE>::STAGES >= 2) {
+    constexpr int NSEG = TM * TN * 4;
+    const OutTile o = vbg_out(a, d, it, post);
+    auto seg = [&](int sg, int& row, int& col, v4f& v) {
+      const int r = sg / (TN * 4), c = (sg / 4) % TN, g = sg % 4;
+      row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
+      col = n0 + wc * WTN + c * 32 + l32;
+      v = v4f{acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
+    };
+    if constexpr (STAGES >= 2) {
     if (nk > 0) {
       load(la0, lb0, 0);
       store<LA>(la0, lds, PLANE_A, sA);
@@ -281,6 +370,7 @@ struct SplitTile {
       step(kt, la1, lb1, la0, lb0);
       if (kt + 1 < nk) step(kt + 1, la0, lb0, la1, lb1);
     }
+    vbg_epilogue<NSEG>(o, d.M, d.N, seg);
     } else {
     // one register stage: tile kt+1 is loaded while tile kt computes and is
     // quantised into the other LDS buffer after the MFMAs
@@ -293,38 +383,44 @@ struct SplitTile {
       store<LB>(lb0, lds + PLANES * PLANE_A, PLANE_B, sB);
     }
     __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
+    v4f cpre[EpiChunk<NSEG>::CH];
+    auto ktile = [&](int kt, auto last, auto pre) {
       const int cur = LDS_STAGES == 2 ? (kt & 1) : 0;
-      const bool more = kt + 1 < nk;
-      if (more) load(la0, lb0, kt + 1);
+      if constexpr (!decltype(last)::value) load(la0, lb0, kt + 1);
+      if constexpr (decltype(pre)::value) vbg_c_load<NSEG>(o, d.M, d.N, seg, 0, cpre);
       compute(lds + cur * STAGE, [](int) {});
       if (LDS_STAGES == 1) __syncthreads();  // every wave is done reading the buffer
-      if (more) {
+      if constexpr (!decltype(last)::value) {
         char* other = lds + (LDS_STAGES == 2 ? (cur ^ 1) : 0) * STAGE;
         store<LA>(la0, other, PLANE_A, sA);
         store<LB>(lb0, other + PLANES * PLANE_A, PLANE_B, sB);
       }
       __syncthreads();
+    };
+    for (int kt = 0; kt + 1 < nk; ++kt) ktile(kt, std::false_type{}, std::false_type{});
+    if constexpr (CPRE) {
+      if (nk > 0) {
+        ktile(nk - 1, std::true_type{}, std::true_type{});
+        vbg_epilogue<NSEG>(o, d.M, d.N, seg, cpre);
+      } else {
+        vbg_epilogue<NSEG>(o, d.M, d.N, seg);
+      }
+    } else {
+      if (nk > 0) ktile(nk - 1, std::true_type{}, std::false_type{});
+      vbg_epilogue<NSEG>(o, d.M, d.N, seg);
     }
     }
-    const OutTile o = vbg_out(a, d, it, post);
-    vbg_epilogue<TM * TN * 4>(o, d.M, d.N, [&](int sg, int& row, int& col, v4f& v) {
-      const int r = sg / (TN * 4), c = (sg / 4) % TN, g = sg % 4;
-      row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
-      col = n0 + wc * WTN + c * 32 + l32;
-      v = v4f{acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
-    });
   }
 };
 
 // Mixed launch: items of the 64x64 .. 128x128 shapes in one persistent grid,
 // each naming its own (wave-uniform) shape; LDS sized for the largest.
-template <int MODE, bool TA, bool TB>
+template <int MODE, bool TA, bool TB, bool PK = false>
 struct SplitMixed {
-  using Sq128 = SplitTile<128, 128, MODE, TA, TB>;
-  using Tall = SplitTile<128, 64, MODE, TA, TB>;
-  using Wide = SplitTile<64, 128, MODE, TA, TB>;
-  using Sq64 = SplitTile<64, 64, MODE, TA, TB>;
+  using Sq128 = SplitTile<128, 128, MODE, TA, TB, false, PK>;
+  using Tall = SplitTile<128, 64, MODE, TA, TB, false, PK>;
+  using Wide = SplitTile<64, 128, MODE, TA, TB, false, PK>;
+  using Sq64 = SplitTile<64, 64, MODE, TA, TB, false, PK>;
   static constexpr int MODE_ID = MODE;
   static constexpr int WAVES = 2;  // two workgroups per CU, as the 128x128 body alone
   static constexpr int LDS_FLOATS = Sq128::LDS_FLOATS;
@@ -345,6 +441,103 @@ __global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_e
 vbg_split_kernel(vbg_args a) {
   __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
   vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
+}
+
+// ---------------------------------------------------------------- operand pack
+// Operand pack pre-pass: job j quantises rows [row0, row0 + 64) x k [k0, k0 +
+// VBG_PACK_K) of one operand into the packed layout PkLoader reads -- rows of
+// Kp = K rounded up to VBG_PACK_KALIGN, k contiguous, zeros past K -- with the
+// same arithmetic as the in-loop quantisation (bf16x3 split, x * scale to FP16
+// or e4m3), so a packed run gives the same bits as an unpacked one.  Operands
+// stored with k contiguous are converted in place order; the others are
+// transposed through LDS, 64 x 64 at a time.  Thread t owns row t / 4 and 16
+// consecutive k of each 64-wide step.
+template <int MODE>
+__global__ void __launch_bounds__(VBG_THREADS)
+    vbg_pack_kernel(const vbg_pack_job* jobs, const vbg_mat_desc* desc, const float* const* A,
+                    const float* const* B, int ta, int tb, const float* scales, char* pk,
+                    const int64_t* pk_off) {
+  constexpr int EB = Mode<MODE>::EB;
+  __shared__ float tr[64][65];
+  const vbg_pack_job j = jobs[blockIdx.x];
+  const vbg_mat_desc d = desc[j.mat];
+  const bool op_a = j.op == 0;
+  const int rows = op_a ? d.M : d.N;
+  const bool kc = op_a ? ta != 0 : tb == 0;  // stored with k contiguous
+  const int ld = op_a ? d.lda : d.ldb;
+  const float* p = op_a ? A[j.mat] : B[j.mat];
+  const float s = MODE == MODE_BF16X3 ? 1.0f : scales[4 * j.mat + j.op];
+  const int Kp = (d.K + VBG_PACK_KALIGN - 1) & ~(VBG_PACK_KALIGN - 1);
+  char* out = pk + pk_off[2 * j.mat + j.op];
+  const int64_t plane = (int64_t)rows * Kp * EB;
+  // the stored operand ends after its last element: rows x K at ld (k
+  // contiguous) or K columns of `rows` at ld; past that, buffer loads read 0
+  const uint32_t extent = kc ? ((uint32_t)(rows - 1) * (uint32_t)ld + (uint32_t)d.K) * 4u
+                             : ((uint32_t)(d.K - 1) * (uint32_t)ld + (uint32_t)rows) * 4u;
+  const __amdgpu_buffer_rsrc_t rs = vbg_rsrc(p, extent);
+  const int t = threadIdx.x;
+  const int orow = t >> 2, oq = (t & 3) * 16;
+  const int r = j.row0 + orow;
+  const int kend = min(Kp, j.k0 + VBG_PACK_K);
+  for (int kb = j.k0; kb < kend; kb += 64) {
+    float x[16];
+    // Addresses use k clamped to the last column (k >= K is zeroed by the
+    // selects), so no offset leaves the operand's 32-bit range.
+    if (kc) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = kb + oq + 4 * q;
+        const v4f v = __builtin_bit_cast(
+            v4f, __builtin_amdgcn_raw_buffer_load_b128(rs, (min(r, rows - 1) * ld + min(k, d.K - 1)) * 4, 0, 0));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[4 * q + e] = k + e < d.K ? v[e] : 0.0f;
+      }
+    } else {
+      __syncthreads();  // the previous step's reads of tr are done
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int kk = (t >> 4) + 16 * i, rr = 4 * (t & 15);
+        const int k = kb + kk;
+        const v4f v = __builtin_bit_cast(
+            v4f, __builtin_amdgcn_raw_buffer_load_b128(rs, (min(k, d.K - 1) * ld + j.row0 + rr) * 4, 0, 0));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tr[kk][rr + e] = k < d.K ? v[e] : 0.0f;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < 16; ++e) x[e] = tr[oq + e][orow];
+    }
+    if (r >= rows) continue;
+    char* dst = out + (int64_t)r * Kp * EB + (int64_t)(kb + oq) * EB;
+    if (MODE == MODE_BF16X3) {
+      v4bf hi[4], lo[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        bf16x3_split4(v4f{x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]}, hi[q], lo[q]);
+      *(v8bf*)dst = __builtin_shufflevector(hi[0], hi[1], 0, 1, 2, 3, 4, 5, 6, 7);
+      *(v8bf*)(dst + 16) = __builtin_shufflevector(hi[2], hi[3], 0, 1, 2, 3, 4, 5, 6, 7);
+      *(v8bf*)(dst + plane) = __builtin_shufflevector(lo[0], lo[1], 0, 1, 2, 3, 4, 5, 6, 7);
+      *(v8bf*)(dst + plane + 16) = __builtin_shufflevector(lo[2], lo[3], 0, 1, 2, 3, 4, 5, 6, 7);
+    } else if (MODE == MODE_FP16) {
+      v8h h0, h1;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        h0[e] = (_Float16)(x[e] * s);
+        h1[e] = (_Float16)(x[8 + e] * s);
+      }
+      *(v8h*)dst = h0;
+      *(v8h*)(dst + 16) = h1;
+    } else {
+      v4u32_t w;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int pq = __builtin_amdgcn_cvt_pk_fp8_f32(x[4 * q] * s, x[4 * q + 1] * s, 0, false);
+        pq = __builtin_amdgcn_cvt_pk_fp8_f32(x[4 * q + 2] * s, x[4 * q + 3] * s, pq, true);
+        w[q] = (unsigned)pq;
+      }
+      *(v4u32_t*)dst = w;
+    }
+  }
 }
 
 // ---------------------------------------------------------------- amax / scales
@@ -474,6 +667,27 @@ static int dispatch_tile(int tile, bool ta, bool tb, const vbg_args* a, int grid
   }
 }
 
+template <int MODE>
+static int dispatch_tile_pk(int tile, const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  switch (tile) {
+    case T64: return run_or_occ<SplitTile<64, 64, MODE, false, false, true, true>>(a, grid, s, occ);
+    case T128x64: return run_or_occ<SplitTile<128, 64, MODE, false, false, true, true>>(a, grid, s, occ);
+    case T64x128: return run_or_occ<SplitTile<64, 128, MODE, false, false, true, true>>(a, grid, s, occ);
+    case T128: return run_or_occ<SplitTile<128, 128, MODE, false, false, true, true>>(a, grid, s, occ);
+    case T_MIXED: return run_or_occ<SplitMixed<MODE, false, false, true>>(a, grid, s, occ);
+    default: return -1;
+  }
+}
+
+static int split_pk_dispatch(int prec, int tile, const vbg_args* a, int grid, hipStream_t s, int* occ) {
+  switch (prec) {
+    case VBG_PREC_BF16X3: return dispatch_tile_pk<MODE_BF16X3>(tile, a, grid, s, occ);
+    case VBG_PREC_FP16_SCALED: return dispatch_tile_pk<MODE_FP16>(tile, a, grid, s, occ);
+    case VBG_PREC_FP8_E4M3: return dispatch_tile_pk<MODE_FP8>(tile, a, grid, s, occ);
+    default: return -1;
+  }
+}
+
 static int split_dispatch(int prec, int tile, bool ta, bool tb, const vbg_args* a, int grid,
                           hipStream_t s, int* occ) {
   switch (prec) {
@@ -492,6 +706,39 @@ int vbg_occupancy_split(int prec, int tile, int ta, int tb) {
   int occ = 1;
   split_dispatch(prec, tile, ta != 0, tb != 0, nullptr, 0, nullptr, &occ);
   return occ;
+}
+
+int vbg_launch_split_pk(int tile, const vbg_args* a, int grid, void* stream) {
+  return split_pk_dispatch(a->prec, tile, a, grid, (hipStream_t)stream, nullptr);
+}
+
+int vbg_occupancy_split_pk(int prec, int tile) {
+  int occ = 1;
+  split_pk_dispatch(prec, tile, nullptr, 0, nullptr, &occ);
+  return occ;
+}
+
+int vbg_launch_pack(int prec, int ta, int tb, const vbg_pack_job* jobs, int n_jobs,
+                    const vbg_mat_desc* desc, const float* const* A, const float* const* B,
+                    const float* scales, char* pk, const int64_t* pk_off, void* stream) {
+  if (n_jobs <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  switch (prec) {
+    case VBG_PREC_BF16X3:
+      hipLaunchKernelGGL(vbg_pack_kernel<MODE_BF16X3>, dim3(n_jobs), dim3(VBG_THREADS), 0, s, jobs, desc, A, B,
+                         ta, tb, scales, pk, pk_off);
+      break;
+    case VBG_PREC_FP16_SCALED:
+      hipLaunchKernelGGL(vbg_pack_kernel<MODE_FP16>, dim3(n_jobs), dim3(VBG_THREADS), 0, s, jobs, desc, A, B,
+                         ta, tb, scales, pk, pk_off);
+      break;
+    case VBG_PREC_FP8_E4M3:
+      hipLaunchKernelGGL(vbg_pack_kernel<MODE_FP8>, dim3(n_jobs), dim3(VBG_THREADS), 0, s, jobs, desc, A, B,
+                         ta, tb, scales, pk, pk_off);
+      break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int vbg_launch_scales(int prec, int batch, int ta, int tb, const vbg_mat_desc* desc,

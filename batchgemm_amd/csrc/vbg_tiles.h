@@ -281,46 +281,68 @@ __device__ __forceinline__ void vbg_store4(float* C, int ldc, int row, int col, 
 // handled in chunks of 8 whose C loads (beta != 0) are all issued before any
 // store, so a tile costs two memory round trips instead of one per segment.
 // A segment crossing row M is written element by element.
-template <int NSEG, class Seg>
-__device__ __forceinline__ void vbg_epilogue(const OutTile& o, int M, int N, Seg&& seg) {
-  const __amdgpu_buffer_rsrc_t rs = vbg_rsrc(o.p, (uint32_t)o.ld * (uint32_t)N * 4u);
-  constexpr int CH = NSEG < 8 ? NSEG : 8;
+// C segments are handled in chunks of up to 8 (32 registers of old values).
+template <int NSEG>
+struct EpiChunk {
+  static constexpr int CH = NSEG < 8 ? NSEG : 8;
   static_assert(NSEG % CH == 0, "segments must split into whole chunks");
+};
+
+// Loads the old C values of segments [c0, c0 + CH) (zero where beta == 0 or
+// outside the matrix): 128-bit buffer loads for whole segments, per-element
+// loads at the M edge.
+template <int NSEG, class Seg>
+__device__ __forceinline__ void vbg_c_load(const OutTile& o, int M, int N, Seg&& seg, int c0,
+                                           v4f (&old)[EpiChunk<NSEG>::CH]) {
+  constexpr int CH = EpiChunk<NSEG>::CH;
+  const __amdgpu_buffer_rsrc_t rs = vbg_rsrc(o.p, (uint32_t)o.ld * (uint32_t)N * 4u);
+#pragma unroll
+  for (int j = 0; j < CH; ++j) {
+    int row, col;
+    v4f v;
+    seg(c0 + j, row, col, v);
+    old[j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    if (o.beta != 0.0f && col < N && row < M) {
+      if (row + 3 < M) {
+        old[j] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rs, (col * o.ld + row) * 4, 0, 0));
+      } else {
+        const float* q = o.p + (size_t)col * o.ld + row;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (row + e < M) old[j][e] = q[e];
+      }
+    }
+  }
+}
+
+// C <- alpha * acc + beta * C over the tile's segments.  All C loads of a
+// chunk are issued before any of its stores.  With PRE, `pre` holds the first
+// chunk's old values, loaded earlier (vbg_c_load) to overlap that round trip
+// with the last k-tile.
+template <int NSEG, bool PRE, class Seg>
+__device__ __forceinline__ void vbg_epilogue_impl(const OutTile& o, int M, int N, Seg&& seg,
+                                                  const v4f (&pre)[EpiChunk<NSEG>::CH]) {
+  constexpr int CH = EpiChunk<NSEG>::CH;
+  const __amdgpu_buffer_rsrc_t rs = vbg_rsrc(o.p, (uint32_t)o.ld * (uint32_t)N * 4u);
 #pragma unroll
   for (int c0 = 0; c0 < NSEG; c0 += CH) {
     v4f old[CH];
-    int off[CH];
-    bool full[CH], any[CH];
+    if (PRE && c0 == 0) {
 #pragma unroll
-    for (int j = 0; j < CH; ++j) {
-      int row, col;
-      v4f v;
-      seg(c0 + j, row, col, v);
-      any[j] = col < N && row < M;
-      full[j] = any[j] && row + 3 < M;
-      off[j] = (col * o.ld + row) * 4;
-      old[j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-      if (o.beta != 0.0f && any[j]) {
-        if (full[j]) {
-          old[j] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rs, off[j], 0, 0));
-        } else {
-          const float* q = o.p + (size_t)col * o.ld + row;
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (row + e < M) old[j][e] = q[e];
-        }
-      }
+      for (int j = 0; j < CH; ++j) old[j] = pre[j];
+    } else {
+      vbg_c_load<NSEG>(o, M, N, seg, c0, old);
     }
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
       int row, col;
       v4f v;
       seg(c0 + j, row, col, v);
-      if (!any[j]) continue;
+      if (!(col < N && row < M)) continue;
       v4f r = v * o.alpha;
       if (o.beta != 0.0f) r += o.beta * old[j];
-      if (full[j]) {
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32_t, r), rs, off[j], 0, 0);
+      if (row + 3 < M) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32_t, r), rs, (col * o.ld + row) * 4, 0, 0);
       } else {
         // Edge segments go through plain global accesses: per-element dword
         // buffer stores of a vector's lanes were compiled to store lane 0 to
@@ -333,4 +355,16 @@ __device__ __forceinline__ void vbg_epilogue(const OutTile& o, int M, int N, Seg
       }
     }
   }
+}
+
+template <int NSEG, class Seg>
+__device__ __forceinline__ void vbg_epilogue(const OutTile& o, int M, int N, Seg&& seg,
+                                             const v4f (&pre)[EpiChunk<NSEG>::CH]) {
+  vbg_epilogue_impl<NSEG, true>(o, M, N, seg, pre);
+}
+
+template <int NSEG, class Seg>
+__device__ __forceinline__ void vbg_epilogue(const OutTile& o, int M, int N, Seg&& seg) {
+  v4f unused[EpiChunk<NSEG>::CH];
+  vbg_epilogue_impl<NSEG, false>(o, M, N, seg, unused);
 }

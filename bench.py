@@ -41,7 +41,7 @@ def time_launches(fn, steps, warmup):
     return s.elapsed_time(e) / steps  # ms
 
 
-KERNEL_TEMPLATE = {1: "Fp32MfmaTile", 2: "SeqTile", 3: "SplitTile", 4: "Bf16Tile"}
+KERNEL_TEMPLATE = {1: "Fp32MfmaTile", 2: "SeqTile", 3: "SplitTile", 4: "Bf16Tile", 7: "SplitTile"}
 
 
 def kernel_name(bg, prec, kind, tile):
@@ -52,12 +52,16 @@ def kernel_name(bg, prec, kind, tile):
         return "vbg_amax_kernel+vbg_scales_kernel"
     if kind == 5:
         return "vbg_splitk_reduce"
+    if kind == 6:
+        return f"vbg_pack_kernel[{prec}]"
+    tag = {3: f"[{prec}]", 7: f"[{prec},packed]"}.get(kind, "")
     if tile == bg._lib.TILE_MIXED:  # one launch over the 64x64 .. 128x128 shapes
-        return {1: "Fp32Mixed", 3: f"SplitMixed[{prec}]"}.get(kind, f"{KERNEL_TEMPLATE[kind]}<mixed>")
+        return {1: "Fp32Mixed", 3: f"SplitMixed{tag}", 7: f"SplitMixed{tag}"}.get(
+            kind, f"{KERNEL_TEMPLATE[kind]}<mixed>")
     bm, bn = bg.TILE_SHAPES[tile]
     if kind == 1 and bm == bn and bm <= 32:
         return f"Fp32SmallTile<{bm}>"
-    return f"{KERNEL_TEMPLATE[kind]}<{bm},{bn}>" + (f"[{prec}]" if kind == 3 else "")
+    return f"{KERNEL_TEMPLATE[kind]}<{bm},{bn}>" + tag
 
 
 def read_traffic(kernel):
@@ -164,20 +168,29 @@ def extras(bg, w, vb, M, N, K, alpha, beta, flops, fp32_value, args):
     # entries scaled by 1e3 (stresses the per-matrix scaling); error of the
     # sampled matrices against FP64.
     vb.fill("normal", seed=args.seed + 1, outlier_frac=0.01, outlier_scale=1e3, c_dist="uniform")
+    # The split precisions run both ways: operands quantised once per run into
+    # a packed workspace ("packed", FP16 / FP8 default on this batch), and
+    # quantised on the HBM -> LDS path ("in_loop"); the two give the same bits.
     prec_out = {}
     for prec in ("fp32", "bf16x3", "fp16", "fp8"):
-        p = bg.Plan(M, N, K, prec=prec)
-        ms = time_launches(lambda: p.run(vb.dA, vb.dB, vb.dC, alpha, beta), max(3, args.steps // 2),
-                           args.warmup)
-        C0 = vb.C.clone()
-        p.run(vb.dA, vb.dB, vb.dC, alpha, beta)
-        torch.cuda.synchronize()
-        err = rel_err_sample(w, vb, M, N, K, alpha, beta, C0, idx)
-        prec_out[prec] = {"gflops": round(flops / (ms * 1e-3) / 1e9, 1), "ms": round(ms, 4),
-                          "rel_err_vs_fp64": float(f"{err:.3g}")}
-        log(f"[bench] outlier inputs {prec}: {ms:.3f} ms, {prec_out[prec]['gflops']} GFLOP/s, "
-            f"rel err {err:.3g}")
-        del p
+        variants = [(prec, {})] if prec == "fp32" else [
+            (f"{prec}_packed", {"pack_operands": 1}), (f"{prec}_in_loop", {"pack_operands": -1})]
+        for key, o in variants:
+            p = bg.Plan(M, N, K, prec=prec, **o)
+            ms = time_launches(lambda: p.run(vb.dA, vb.dB, vb.dC, alpha, beta), max(3, args.steps // 2),
+                               args.warmup)
+            C0 = vb.C.clone()
+            p.run(vb.dA, vb.dB, vb.dC, alpha, beta)
+            torch.cuda.synchronize()
+            err = rel_err_sample(w, vb, M, N, K, alpha, beta, C0, idx)
+            prec_out[key] = {"gflops": round(flops / (ms * 1e-3) / 1e9, 1), "ms": round(ms, 4),
+                             "rel_err_vs_fp64": float(f"{err:.3g}")}
+            log(f"[bench] outlier inputs {key}: {ms:.3f} ms, {prec_out[key]['gflops']} GFLOP/s, "
+                f"rel err {err:.3g}")
+            del p
+        if prec != "fp32":
+            packed_default = bg.Plan(M, N, K, prec=prec, host_only=True).packed_bytes > 0
+            prec_out[f"{prec}_default"] = "packed" if packed_default else "in_loop"
     out["precision_outliers"] = {"inputs": "N(0,1), 1% of entries x1e3, C ~ U[-1,1]", **prec_out}
     lib_ms = per_matrix_library(vb, M, N, K, alpha, beta, max(3, args.steps // 4), 1)
     out["per_matrix_library_fp32"] = {

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define VBG_ABI_VERSION 1
+#define VBG_ABI_VERSION 2
 
 typedef enum vbg_status {
   VBG_OK = 0,
@@ -113,6 +113,12 @@ typedef struct vbg_opts {
   int consolidate;      /* AUTO: >= 0 (default) moves the matrices of a tile shape whose launch
                            would be short into the dominant launch's shape when that is
                            modelled cheaper than a launch of their own; < 0 off */
+  int pack_operands;    /* bf16x3 / FP16 / FP8: > 0 quantise every operand once per run into a
+                           plan-owned workspace (k-contiguous rows, K padded to 64) that the
+                           GEMM kernels stream without converting; < 0 quantise on the
+                           HBM -> LDS path instead; 0 = default (FP16 / FP8 packed when the
+                           batch's arithmetic intensity, 2*sum(MNK) / 4*sum(MK+KN), is at
+                           least 64 flop/B; bf16x3 unpacked) */
 } vbg_opts;
 
 typedef struct vbg_plan vbg_plan;
@@ -172,7 +178,10 @@ int vbg_sgemm_vbatched_oneshot(int batch, const int* M, const int* N, const int*
  *     issues, in issue order (out[5*rows]); tile id 6 = a mixed launch (items of the 64x64 ..
  *     128x128 shapes, see query 6); kind 0 = scale pre-pass (FP16 / FP8), 1 = FP32
  *     MFMA tiles, 2 = FP32 sequential-fma tiles, 3 = split-precision tiles, 4 = BF16 tiles,
- *     5 = split-K reduction; items index the lists of queries 4 and 6
+ *     5 = split-K reduction, 6 = operand pack pre-pass (grid = pack jobs), 7 = split-precision
+ *     tiles on packed operands; items index the lists of queries 4 and 6
+ *   8 bytes of the packed-operand workspace (0 when the plan does not pack), as the return
+ *     value, saturated at INT_MAX
  * Returns the number of ints written (or the scalar for 0, 1, 3). */
 int vbg_plan_query(const vbg_plan* plan, int what, int* out, int out_len);
 

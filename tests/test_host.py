@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol(bg):
     missing = [s for s in syms if not hasattr(L, s)]
     assert not missing, missing
     assert sorted(syms) == sorted(bg._lib.EXPORTS)
-    assert bg.lib().vbg_abi_version() == 1
+    assert bg.lib().vbg_abi_version() == 2
 
 
 def test_status_strings(bg):
@@ -175,7 +175,12 @@ def test_kernel_table_and_item_flops(bg):
         kinds = list(kt[:, 0])
         assert (kinds[0] == 0) == (prec == "fp16")
         assert (kinds[-1] == 5) == (ks > 1)
-        body = kt[[k not in (0, 5) for k in kinds]]
+        packed = p.packed_bytes > 0
+        assert (6 in kinds) == packed
+        if packed:  # the pack pre-pass follows the scale pre-pass, before every GEMM launch
+            assert kinds.index(6) == (1 if prec == "fp16" else 0)
+            assert all(k == 7 for k in kinds if k not in (0, 5, 6))
+        body = kt[[k not in (0, 5, 6) for k in kinds]]
         assert len(body) == p.n_launches
         assert body[0, 2] == 0 and body[:, 3].sum() == p.n_items
         assert np.array_equal(body[1:, 2], np.cumsum(body[:, 3])[:-1])
@@ -193,6 +198,32 @@ def test_kernel_table_and_item_flops(bg):
         with pytest.raises(VbgError) as e:
             p.kernel_times()
         assert e.value.code == -5
+
+
+def test_packed_operand_workspace(bg):
+    """Packed operands: per matrix, A (M rows) then B (N rows) of K rounded up
+    to 64 elements at 4 / 2 / 1 bytes (bf16x3 hi+lo / FP16 / FP8), each
+    256-byte aligned; by default FP16 / FP8 pack a high-intensity batch (the
+    unbalanced one) and not a low-intensity one, bf16x3 never;
+    pack_operands forces either way."""
+    from batchgemm_amd import workloads as w
+    M, N, K = w.shapes_unbalanced(300, seed=4)
+    M[3] = 0  # an empty matrix and a K = 0 one hold no packed operands
+    K[5] = 0
+    for prec, eb in (("bf16x3", 4), ("fp16", 2), ("fp8", 1)):
+        kp = -(-K.astype(np.int64) // 64) * 64
+        live = (M > 0) & (N > 0) & (K > 0)
+        al = lambda x: -(-x // 256) * 256
+        want = int(np.sum(np.where(live, al(M * kp * eb) + al(N * kp * eb), 0)))
+        p = bg.Plan(M, N, K, host_only=True, prec=prec, pack_operands=1)
+        assert p.packed_bytes == want, prec
+        dflt = bg.Plan(M, N, K, host_only=True, prec=prec).packed_bytes
+        assert dflt == (0 if prec == "bf16x3" else want), prec
+        assert bg.Plan(M, N, K, host_only=True, prec=prec, pack_operands=-1).packed_bytes == 0
+    Ms, Ns, Ks = [np.full(64, 40, np.int32)] * 3  # AI = 2*40^3 / (4*2*40^2) = 10 flop/B
+    assert bg.Plan(Ms, Ns, Ks, host_only=True, prec="fp8").packed_bytes == 0
+    assert bg.Plan(Ms, Ns, Ks, host_only=True, prec="fp8", pack_operands=1).packed_bytes > 0
+    assert bg.Plan(Ms, Ns, Ks, host_only=True, prec="fp32", pack_operands=1).packed_bytes == 0
 
 
 def test_driver_gflop_count_matches_reference_logs(bg):
