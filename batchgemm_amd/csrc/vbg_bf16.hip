@@ -29,30 +29,46 @@ __device__ __forceinline__ int bswz(int row, int byte) {
 
 // BF16 operand loader.  LDS plane layout [mn][k], k contiguous, PITCH bytes.
 // KC (k contiguous in HBM): a lane loads 8 consecutive k of one row.
-// MC (mn contiguous in HBM): a lane loads 8 consecutive mn at k and at k+1,
-// then interleaves them into 8 (k, k+1) pairs, one 4-byte LDS write per row.
+// MC (mn contiguous in HBM): a lane owns a block of 4 consecutive mn x KB
+// consecutive k (KB = 8 for 128-row tiles, 4 for 64-row ones): KB 8-byte
+// loads (4 mn each, one per k), a 16-bit transpose in registers (one byte
+// permute per output word), then 4 LDS writes of KB k each (16 or 8 bytes).
+// Lanes 0..31 of a wave cover 128 consecutive mn at one k: 256-byte loads.
 // Buffer loads: a lane's byte offset is fixed once per work item and the
 // k-tile position is a wave-uniform scalar offset.  Rows past MN are clamped
 // (they only feed outputs that are never stored); k past K is zero; a vector
 // that straddles MN or K falls back to 2-byte loads.
 template <int ROWS, int BK, int PITCH, bool KC>
 struct Bf16Loader {
-  static constexpr int UNITS = KC ? ROWS * BK / 8 : ROWS * BK / 16;
-  static constexpr int NV = UNITS / VBG_THREADS;
-  static_assert(NV >= 1 && UNITS % VBG_THREADS == 0, "tile too small for the block");
-  typedef v4u Regs[KC ? NV : 2 * NV];
+  static constexpr int KB = ROWS * BK / (4 * VBG_THREADS);  // MC: k per lane
+  static_assert(KC || KB == 4 || KB == 8, "MC block of 4 mn x 4 or 8 k per lane");
+  static constexpr int NV = KC ? ROWS * BK / 8 / VBG_THREADS : 1;
+  static_assert(!KC || (NV >= 1 && ROWS * BK % (8 * VBG_THREADS) == 0), "tile too small for the block");
+  typedef unsigned v2u_t __attribute__((ext_vector_type(2)));
+  struct Regs {
+    v4u kc[KC ? NV : 1];
+    v2u_t mc[KC ? 1 : KB];
+  };
   __amdgpu_buffer_rsrc_t rsrc;
   int voff[NV];
   int ld2, MN, mn0;
 
+  // MC lane map: each 8-lane group takes 4 consecutive mn-blocks (t = lane %
+  // 4) at 2 consecutive k-groups (lane / 4 % 2).  Its 16-byte LDS writes of
+  // one mn offset then fall in 8 distinct swizzled units (unit = kg ^ (row /
+  // 2) over rows 4*blk + q), so they use 8 distinct bank groups; the 32 lanes
+  // of one k-group still cover 4 * ROWS / 16 * 4 consecutive mn (256 or 128
+  // contiguous bytes per load instruction).
   __device__ __forceinline__ static void coords(int v, int& row, int& k) {
     const int idx = threadIdx.x + VBG_THREADS * v;
     if (KC) {
       row = idx / (BK / 8);
       k = 8 * (idx % (BK / 8));
     } else {
-      row = 8 * (idx % (ROWS / 8));
-      k = 2 * (idx / (ROWS / 8));
+      constexpr int BPW = ROWS / 16;  // 4-block groups along mn
+      const int l = idx & 63, w = idx >> 6, g = l >> 3, s = l & 7;
+      row = 4 * (4 * (g % BPW) + (s & 3));
+      k = KB * (2 * (w * (8 / BPW) + g / BPW) + (s >> 2));
     }
   }
 
@@ -80,16 +96,19 @@ struct Bf16Loader {
   __device__ __forceinline__ v4u ld128(int off, int soff) const {
     return __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, soff, 0));
   }
+  __device__ __forceinline__ v2u_t ld64(int off, int soff) const {
+    return __builtin_bit_cast(v2u_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, soff, 0));
+  }
 
   __device__ __forceinline__ void fetch(Regs& r, int k0, int K) const {
+    if constexpr (KC) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      int row, kk;
-      coords(v, row, kk);
-      const int k = k0 + kk;
-      if (KC) {
+      for (int v = 0; v < NV; ++v) {
+        int row, kk;
+        coords(v, row, kk);
+        const int k = k0 + kk;
         if (k + 8 <= K) {
-          r[v] = ld128(voff[v], k0 * 2);
+          r.kc[v] = ld128(voff[v], k0 * 2);
         } else {
           u32 w[4];
 #pragma unroll
@@ -98,48 +117,56 @@ struct Bf16Loader {
             const u32 hi = k + 2 * e + 1 < K ? ld16(voff[v] + 4 * e + 2, k0 * 2) : 0u;
             w[e] = lo | (hi << 16);
           }
-          r[v] = v4u{w[0], w[1], w[2], w[3]};
+          r.kc[v] = v4u{w[0], w[1], w[2], w[3]};
         }
-      } else {
-        const int mn = mn0 + row;
+      }
+    } else {
+      int row, kk;
+      coords(0, row, kk);
+      const int mn = mn0 + row, k = k0 + kk;
+      const bool whole = mn + 3 < MN;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int soff = (k0 + j) * ld2;
-          v4u x;
-          if (mn + 7 < MN) {
-            x = ld128(voff[v], soff);
-          } else {  // straddles MN: element by element, clamped
-            u32 w[4];
+      for (int j = 0; j < KB; ++j) {
+        const int soff = (k0 + j) * ld2;
+        v2u_t x;
+        if (whole) {
+          x = ld64(voff[0], soff);
+        } else {  // straddles MN: element by element, clamped
+          const int c0 = min(mn, MN - 1);
+          u32 e[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int a = min(mn + 2 * e, MN - 1) - min(mn, MN - 1);
-              const int b = min(mn + 2 * e + 1, MN - 1) - min(mn, MN - 1);
-              w[e] = ld16(voff[v] + 2 * a, soff) | (ld16(voff[v] + 2 * b, soff) << 16);
-            }
-            x = v4u{w[0], w[1], w[2], w[3]};
-          }
-          if (k + j >= K) x = v4u{0u, 0u, 0u, 0u};
-          r[2 * v + j] = x;
+          for (int q = 0; q < 4; ++q) e[q] = ld16(voff[0] + 2 * (min(mn + q, MN - 1) - c0), soff);
+          x = v2u_t{e[0] | (e[1] << 16), e[2] | (e[3] << 16)};
         }
+        if (k + j >= K) x = v2u_t{0u, 0u};
+        r.mc[j] = x;
       }
     }
   }
 
   __device__ __forceinline__ static void store(const Regs& r, char* plane) {
+    if constexpr (KC) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
+      for (int v = 0; v < NV; ++v) {
+        int row, kk;
+        coords(v, row, kk);
+        *(v4u*)(plane + bswz<PITCH>(row, 2 * kk)) = r.kc[v];
+      }
+    } else {
       int row, kk;
-      coords(v, row, kk);
-      if (KC) {
-        *(v4u*)(plane + bswz<PITCH>(row, 2 * kk)) = r[v];
-      } else {
-        const v4u a = r[2 * v], b = r[2 * v + 1];  // k, k+1; 8 mn each
+      coords(0, row, kk);
+      // element q of the 4-mn vector at k + j is 16-bit half q % 2 of word q / 2
+      auto pair = [&](int j, int q) -> u32 {
+        const u32 a = r.mc[j][q >> 1], b = r.mc[j + 1][q >> 1];
+        return (q & 1) ? __builtin_amdgcn_perm(b, a, 0x07060302u) : __builtin_amdgcn_perm(b, a, 0x05040100u);
+      };
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const u32 even = (a[e] & 0xffffu) | (b[e] << 16);
-          const u32 odd = (a[e] >> 16) | (b[e] & 0xffff0000u);
-          *(u32*)(plane + bswz<PITCH>(row + 2 * e, 2 * kk)) = even;
-          *(u32*)(plane + bswz<PITCH>(row + 2 * e + 1, 2 * kk)) = odd;
+      for (int q = 0; q < 4; ++q) {
+        char* dst = plane + bswz<PITCH>(row + q, 2 * kk);
+        if constexpr (KB == 8) {
+          *(v4u*)dst = v4u{pair(0, q), pair(2, q), pair(4, q), pair(6, q)};
+        } else {
+          *(v2u_t*)dst = v2u_t{pair(0, q), pair(2, q)};
         }
       }
     }
@@ -180,6 +207,16 @@ __device__ __forceinline__ void store4_bf16(u16* C, int ldc, int row, int col, i
   }
 }
 
+// Register stages of the k-loop: 2 = tile kt + 2 loads while tile kt + 1 is
+// stored between the MFMA steps of tile kt; 1 = tile kt + 1 loads during tile
+// kt and is stored after it.  Measured on the grouped config (G = 128, K =
+// 2048, N = 1408), two runs each: 2 stages fwd 0.580 / 0.581 ms, bwd 1.258 /
+// 1.260; 1 stage fwd 0.591 / 0.588, bwd 1.241 / 1.240 -- a wash (the loop is
+// bound by LDS instruction issue, not load latency), so the simpler one.
+#ifndef VBG_BF16_STAGES
+#define VBG_BF16_STAGES 1
+#endif
+
 template <int BM, int BN, bool TA, bool TB, bool OUT_BF16>
 struct Bf16Tile {
   static constexpr int BK = 64;
@@ -219,24 +256,7 @@ struct Bf16Tile {
     LB lb;
     la.init(A, d.lda, m0, d.M, d.K);
     lb.init(B, d.ldb, n0, d.N, d.K);
-    typename LA::Regs ra;
-    typename LB::Regs rb;
-    if (nk > 0) {
-      la.fetch(ra, kb, ke);
-      lb.fetch(rb, kb, ke);
-      LA::store(ra, lds);
-      LB::store(rb, lds + PLANE_A);
-    }
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-      const int cur = kt & 1;
-      const bool more = kt + 1 < nk;
-      if (more) {
-        const int k0 = kb + (kt + 1) * BK;
-        la.fetch(ra, k0, ke);
-        lb.fetch(rb, k0, ke);
-      }
-      const char* As = lds + cur * STAGE;
+    auto compute = [&](const char* As, auto&& mid) {
       const char* Bs = As + PLANE_A;
 #pragma unroll
       for (int s = 0; s < BK / 16; ++s) {
@@ -251,13 +271,72 @@ struct Bf16Tile {
 #pragma unroll
           for (int c = 0; c < TN; ++c)
             acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[r], bv[c], acc[r][c], 0, 0, 0);
+        mid(s);
       }
-      if (more) {
-        char* nxt = lds + (cur ^ 1) * STAGE;
-        LA::store(ra, nxt);
-        LB::store(rb, nxt + PLANE_A);
+    };
+    if constexpr (VBG_BF16_STAGES >= 2) {
+      // Two register stages: while the MFMAs read LDS buffer kt & 1, tile
+      // kt + 1 (loaded one step earlier) goes into the other buffer between
+      // the MFMA steps and the loads of tile kt + 2 are in flight.
+      typename LA::Regs ra0, ra1;
+      typename LB::Regs rb0, rb1;
+      auto step = [&](int kt, typename LA::Regs& nxa, typename LB::Regs& nxb, typename LA::Regs& lda_,
+                      typename LB::Regs& ldb_) {
+        const int cur = kt & 1;
+        if (kt + 2 < nk) {
+          la.fetch(lda_, kb + (kt + 2) * BK, ke);
+          lb.fetch(ldb_, kb + (kt + 2) * BK, ke);
+        }
+        char* other = lds + (cur ^ 1) * STAGE;
+        const bool more = kt + 1 < nk;
+        compute(lds + cur * STAGE, [&](int s) {
+          if (!more) return;
+          if (s == 0) LA::store(nxa, other);
+          if (s == BK / 32) LB::store(nxb, other + PLANE_A);
+        });
+        __syncthreads();
+      };
+      if (nk > 0) {
+        la.fetch(ra0, kb, ke);
+        lb.fetch(rb0, kb, ke);
+        LA::store(ra0, lds);
+        LB::store(rb0, lds + PLANE_A);
+      }
+      if (nk > 1) {
+        la.fetch(ra1, kb + BK, ke);
+        lb.fetch(rb1, kb + BK, ke);
       }
       __syncthreads();
+      for (int kt = 0; kt < nk; kt += 2) {
+        step(kt, ra1, rb1, ra0, rb0);
+        if (kt + 1 < nk) step(kt + 1, ra0, rb0, ra1, rb1);
+      }
+    } else {
+      typename LA::Regs ra;
+      typename LB::Regs rb;
+      if (nk > 0) {
+        la.fetch(ra, kb, ke);
+        lb.fetch(rb, kb, ke);
+        LA::store(ra, lds);
+        LB::store(rb, lds + PLANE_A);
+      }
+      __syncthreads();
+      for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+          const int k0 = kb + (kt + 1) * BK;
+          la.fetch(ra, k0, ke);
+          lb.fetch(rb, k0, ke);
+        }
+        compute(lds + cur * STAGE, [](int) {});
+        if (more) {
+          char* nxt = lds + (cur ^ 1) * STAGE;
+          LA::store(ra, nxt);
+          LB::store(rb, nxt + PLANE_A);
+        }
+        __syncthreads();
+      }
     }
     const OutTile o = vbg_out(a, d, it, 1.0f);
 #pragma unroll
@@ -277,8 +356,11 @@ struct Bf16Tile {
   }
 };
 
+// Two workgroups per CU (the 64 KB LDS of the 128x128 tile allows no more):
+// the register allocation is held to two waves per SIMD.
 template <class Tile>
-__global__ void __launch_bounds__(VBG_THREADS) vbg_bf16_kernel(vbg_args a) {
+__global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+vbg_bf16_kernel(vbg_args a) {
   __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
   vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
 }

@@ -89,7 +89,13 @@ void choose_auto(const std::vector<Shape>& sh, int prec, const vbg_opts& o, int 
   // Undersubscription repair.  It stops at 64x64: on small batches the
   // 32x32 / 16x16 kernels' per-item latency outweighs the extra parallelism
   // (measured on the reference's dataset at B = 8 / 16: 64x64 items run the
-  // batch in 29 / 34 us, 32x32 items in 51 / 70 us).
+  // batch in 29 / 34 us, 32x32 items in 51 / 70 us).  Matrices with K >=
+  // kRepairMaxK keep their tile when split-K is available: split-K then cuts
+  // their long items instead, at the larger tile's LDS and MFMA efficiency
+  // (measured on the grouped backward's dB pass, one 1408 x 2048 x 32636
+  // matrix: 64x64 tiles 634 us; see the planner notes on split-K).
+  const int kRepairMaxK = 2048;
+  const bool split_ok = prec != VBG_PREC_FP32_SEQ && o.split_k_min >= 0;
   int64_t items = 0;
   for (int b = 0; b < B; ++b) items += num_tiles(tile[b], sh[b]);
   const int smaller[NTILES] = {-1, -1, -1, T64, T64, T128x64};
@@ -97,6 +103,7 @@ void choose_auto(const std::vector<Shape>& sh, int prec, const vbg_opts& o, int 
     int pick = -1;
     double longest = 0;
     for (int b = 0; b < B; ++b) {
+      if (split_ok && sh[b].K >= kRepairMaxK) continue;
       int s = smaller[tile[b]];
       while (s >= 0 && !tile_allowed(prec, s, o.tile_mask)) s = smaller[s];
       if (s < 0) continue;

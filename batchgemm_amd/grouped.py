@@ -118,3 +118,60 @@ def grouped_gemm(A: torch.Tensor, B: torch.Tensor, m_sizes, out_dtype=torch.bflo
     """Differentiable grouped GEMM: rows of A grouped by `m_sizes`, times B[g]."""
     plan = grouped_plan(m_sizes, A.shape[1], B.shape[2])
     return GroupedGemmFunction.apply(A, B, plan, out_dtype, grad_dtype)
+
+
+def bench_config4(steps: int = 20, warmup: int = 3, seed: int = 0) -> dict:
+    """Grouped forward and backward on the differentiation workload: G = 128
+    groups, K = 2048, N = 1408, Zipf(1.2) row counts summing to 32768, BF16
+    operands ~ N(0, 0.02), BF16 dC ~ N(0, 1), BF16 outputs; mean ms per call
+    over `steps` calls after `warmup`, from HIP events on the current stream."""
+    from .workloads import zipf_groups
+    m_sizes = zipf_groups(128, 32768, 1.2, seed=seed)
+    K, N, S = 2048, 1408, 32768
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A = (torch.randn(S, K, generator=g, device="cuda") * 0.02).to(torch.bfloat16)
+    B = (torch.randn(128, K, N, generator=g, device="cuda") * 0.02).to(torch.bfloat16)
+    dC = torch.randn(S, N, generator=g, device="cuda").to(torch.bfloat16)
+    plan = GroupedPlan(m_sizes, K, N)
+    C = torch.empty(S, N, dtype=torch.bfloat16, device="cuda")
+    dA, dB = torch.empty_like(A), torch.empty_like(B)
+    L = lib()
+
+    def fwd():
+        check(L.vbg_grouped_gemm_fwd(plan._p, A.data_ptr(), B.data_ptr(), C.data_ptr(), 1,
+                                     _stream_ptr(None)), "vbg_grouped_gemm_fwd")
+
+    def bwd():
+        check(L.vbg_grouped_gemm_bwd(plan._p, A.data_ptr(), B.data_ptr(), dC.data_ptr(), 1, dA.data_ptr(),
+                                     dB.data_ptr(), 1, _stream_ptr(None)), "vbg_grouped_gemm_bwd")
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    f_ms, b_ms = timed(fwd), timed(bwd)
+    gfl = 2.0 * S * K * N / 1e9
+    return {"workload": "G=128, K=2048, N=1408, Zipf(1.2) rows summing to 32768, BF16 in/out, FP32 accumulate",
+            "fwd_ms": round(f_ms, 4), "bwd_ms": round(b_ms, 4), "fwd_bwd_ms": round(f_ms + b_ms, 4),
+            "fwd_tflops": round(gfl / f_ms, 1), "bwd_tflops": round(2 * gfl / b_ms, 1)}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="grouped BF16 forward/backward timing (differentiation workload)")
+    ap.add_argument("--bench", action="store_true", help="time the G=128, K=2048, N=1408 workload")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    args = ap.parse_args()
+    if not args.bench:
+        ap.error("nothing to do: pass --bench")
+    print(json.dumps(bench_config4(args.steps, args.warmup)), flush=True)

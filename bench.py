@@ -198,35 +198,9 @@ def extras(bg, w, vb, M, N, K, alpha, beta, flops, fp32_value, args):
         "ms": round(lib_ms, 3), "gflops": round(flops / (lib_ms * 1e-3) / 1e9, 1)}
     log(f"[bench] per-matrix library loop: {lib_ms:.3f} ms")
     # grouped forward + backward
-    from batchgemm_amd.grouped import GroupedPlan
-    m_sizes = w.zipf_groups(128, 32768, 1.2, seed=args.seed)
-    Kg, Ng = 2048, 1408
-    g = torch.Generator(device="cuda").manual_seed(args.seed)
-    A = (torch.randn(32768, Kg, generator=g, device="cuda") * 0.02).to(torch.bfloat16)
-    B = (torch.randn(128, Kg, Ng, generator=g, device="cuda") * 0.02).to(torch.bfloat16)
-    dC = torch.randn(32768, Ng, generator=g, device="cuda").to(torch.bfloat16)
-    gp = GroupedPlan(m_sizes, Kg, Ng)
-    C = torch.empty(32768, Ng, dtype=torch.bfloat16, device="cuda")
-    dA = torch.empty_like(A); dB = torch.empty_like(B)
-    L = gp._p
-    lib = bg.lib()
-    from batchgemm_amd.api import _stream_ptr
-    def fwd():
-        bg._lib.check(lib.vbg_grouped_gemm_fwd(L, A.data_ptr(), B.data_ptr(), C.data_ptr(), 1,
-                                               _stream_ptr(None)), "grouped fwd")
-
-    def bwd():
-        bg._lib.check(lib.vbg_grouped_gemm_bwd(L, A.data_ptr(), B.data_ptr(), dC.data_ptr(), 1,
-                                               dA.data_ptr(), dB.data_ptr(), 1, _stream_ptr(None)),
-                      "grouped bwd")
-    f_ms = time_launches(fwd, args.steps, args.warmup)
-    b_ms = time_launches(bwd, args.steps, args.warmup)
-    gfl = 2.0 * 32768 * Kg * Ng
-    out["grouped_bf16"] = {
-        "workload": "G=128, K=2048, N=1408, Zipf(1.2) rows summing to 32768, BF16 in/out, FP32 accumulate",
-        "fwd_ms": round(f_ms, 4), "bwd_ms": round(b_ms, 4), "fwd_bwd_ms": round(f_ms + b_ms, 4),
-        "fwd_tflops": round(gfl / (f_ms * 1e-3) / 1e12, 1),
-        "bwd_tflops": round(2 * gfl / (b_ms * 1e-3) / 1e12, 1)}
+    from batchgemm_amd.grouped import bench_config4
+    out["grouped_bf16"] = bench_config4(args.steps, args.warmup, args.seed)
+    f_ms, b_ms = out["grouped_bf16"]["fwd_ms"], out["grouped_bf16"]["bwd_ms"]
     log(f"[bench] grouped fwd {f_ms:.3f} ms bwd {b_ms:.3f} ms")
     return out
 
