@@ -100,7 +100,26 @@ struct Bf16Loader {
     return __builtin_bit_cast(v2u_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, soff, 0));
   }
 
+  // A whole k-tile inside the operand (k0 + BK <= K and, for MC, every row
+  // of the tile below MN) loads with straight-line code; the decision is
+  // wave-uniform, so the k-loop stays one basic block the scheduler can
+  // interleave with the LDS reads and MFMAs.  Other tiles take the per-lane
+  // edge path.
   __device__ __forceinline__ void fetch(Regs& r, int k0, int K) const {
+    if (k0 + BK <= K && (KC || mn0 + ROWS <= MN)) {
+      if constexpr (KC) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) r.kc[v] = ld128(voff[v], k0 * 2);
+      } else {
+#pragma unroll
+        for (int j = 0; j < KB; ++j) r.mc[j] = ld64(voff[0], (k0 + j) * ld2);
+      }
+    } else {
+      fetch_edge(r, k0, K);
+    }
+  }
+
+  __device__ __forceinline__ void fetch_edge(Regs& r, int k0, int K) const {
     if constexpr (KC) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
@@ -210,9 +229,9 @@ __device__ __forceinline__ void store4_bf16(u16* C, int ldc, int row, int col, i
 // Register stages of the k-loop: 2 = tile kt + 2 loads while tile kt + 1 is
 // stored between the MFMA steps of tile kt; 1 = tile kt + 1 loads during tile
 // kt and is stored after it.  Measured on the grouped config (G = 128, K =
-// 2048, N = 1408), two runs each: 2 stages fwd 0.580 / 0.581 ms, bwd 1.258 /
-// 1.260; 1 stage fwd 0.591 / 0.588, bwd 1.241 / 1.240 -- a wash (the loop is
-// bound by LDS instruction issue, not load latency), so the simpler one.
+// 2048, N = 1408), with the straight-line whole-tile fetch: 1 stage fwd 0.535
+// ms, bwd 1.140; 2 stages fwd 0.551, bwd 1.197 (the dense kernels lose 4-13%,
+// the HBM-bound skinny-group kernels do not move).
 #ifndef VBG_BF16_STAGES
 #define VBG_BF16_STAGES 1
 #endif

@@ -92,8 +92,10 @@ struct Mode;
 #define VBG_SPLIT_FENCE 1
 #endif
 // Register stages on packed operands (no conversion between load and LDS).
+// Measured on the unbalanced B=2048 batch, GEMM kernel ms with 2 / 1 stages:
+// bf16x3 3.97 / 3.58, FP16 2.15 / 2.12, FP8 1.88 / 1.83.
 #ifndef VBG_PK_STAGES
-#define VBG_PK_STAGES 2
+#define VBG_PK_STAGES 1
 #endif
 template <>
 struct Mode<MODE_BF16X3> {

@@ -41,6 +41,23 @@ def time_launches(fn, steps, warmup):
     return s.elapsed_time(e) / steps  # ms
 
 
+def time_synced(fn, steps, warmup):
+    """Mean wall ms of fn() followed by a host synchronisation, per call: the
+    reference's mode of one launch then hipDeviceSynchronize
+    (ROCm/framework/gemm.cuh:246-249), launch and sync latency included."""
+    import torch
+    for _ in range(warmup):
+        fn()
+        torch.cuda.synchronize()
+    t = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        t.append(time.perf_counter() - t0)
+    return 1e3 * float(np.mean(t))
+
+
 KERNEL_TEMPLATE = {1: "Fp32MfmaTile", 2: "SeqTile", 3: "SplitTile", 4: "Bf16Tile", 7: "SplitTile"}
 
 
@@ -162,7 +179,12 @@ def extras(bg, w, vb, M, N, K, alpha, beta, flops, fp32_value, args):
         err = rel_err_sample(w, vb, M, N, K, alpha, beta, C0, idx)
         out[prec] = {"gflops": round(flops / (ms * 1e-3) / 1e9, 1), "ms": round(ms, 4),
                      "rel_err_vs_fp64": float(f"{err:.3g}")}
-        log(f"[bench] {prec}: {ms:.3f} ms, {out[prec]['gflops']} GFLOP/s, rel err {err:.3g}")
+        # one batch at a time, host-synchronised after each (the reference's mode)
+        ms_sync = time_synced(lambda: p.run(vb.dA, vb.dB, vb.dC, alpha, beta), args.steps, args.warmup)
+        out[prec]["sync_per_batch_ms"] = round(ms_sync, 4)
+        out[prec]["sync_per_batch_gflops"] = round(flops / (ms_sync * 1e-3) / 1e9, 1)
+        log(f"[bench] {prec}: {ms:.3f} ms, {out[prec]['gflops']} GFLOP/s, rel err {err:.3g}; "
+            f"synchronised per batch {ms_sync:.3f} ms")
     out["bf16x3"]["speedup_vs_fp32"] = round(out["bf16x3"]["gflops"] / out["fp32"]["gflops"], 3)
     # Precision paths on the same shapes with N(0,1) inputs and 1% of the
     # entries scaled by 1e3 (stresses the per-matrix scaling); error of the
