@@ -12,10 +12,11 @@
 // dB the ragged M_g is the reduction length; M_g = 0 writes a zero dB_g.
 // Per-group pointers are derived on the device from the call's base pointers
 // by a small kernel, so a call allocates nothing and issues no memset.
-// Groups of at most kSmallRows rows take dB from a direct kernel instead: for
-// them dB_g is an outer product of reduction length M_g <= 16, whose cost is
-// writing K x N outputs, not MFMA work -- as GEMM items they would pad M_g up
-// to a 64-deep k-tile in every 128 x 128 output tile.
+// Groups of at most kSmallRows rows take all three passes from direct
+// streaming kernels instead of GEMM items, which would pad M_g up to a 64-wide
+// tile side (forward, dA) or a 64-deep k-tile (dB) in every output tile.  For
+// them each pass costs one sweep over a K x N matrix -- reading B_g (forward,
+// dA) or writing dB_g -- so the kernels are laid out for contiguous HBM rows.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -61,48 +62,250 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
   return __builtin_bit_cast(float, (unsigned)h << 16);
 }
 
-// dB_g[k][n] = sum_{r < M_g} A[off_g + r][k] * dC[off_g + r][n] for the small
-// groups (blockIdx.y indexes them): 8 consecutive n of one k per thread, FP32
-// accumulation in row order, 16- or 32-byte stores.  BF16 products are exact
-// in FP32, so only the additions round.
-template <bool OUT_BF16>
+typedef unsigned short u16;
+
+// Row-count buckets of the small groups: each small-group kernel is built for
+// NR rows (1, 2, 4, 8, 16) and run over the groups whose M_g fits, rows past
+// M_g reading as zero, so its inner loops are straight-line code.
+constexpr int kBuckets = 5;
+__host__ __device__ constexpr int bucket_rows(int b) { return 1 << b; }
+
+__device__ __forceinline__ void bf16x8_to_f32(const uint4 raw, float (&f)[8]) {
+  const unsigned w[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    f[2 * e] = __builtin_bit_cast(float, w[e] << 16);
+    f[2 * e + 1] = __builtin_bit_cast(float, w[e] & 0xffff0000u);
+  }
+}
+
+__device__ __forceinline__ uint4 f32x8_to_bf16(const float (&f)[8]) {
+  unsigned w[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const __bf16 lo = (__bf16)f[2 * e], hi = (__bf16)f[2 * e + 1];
+    w[e] = (unsigned)__builtin_bit_cast(unsigned short, lo) |
+           ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
+  }
+  return uint4{w[0], w[1], w[2], w[3]};
+}
+
+// Forward of the small groups: C[off_g + r][c] = sum_k A[off_g + r][k] *
+// B_g[k][c], r < M_g <= NR.  The cost is streaming B_g (K x N) once.  Block
+// (blockIdx.x, group blockIdx.y): 256 consecutive columns over all of K; lane
+// t takes columns 8 (t % 32) .. + 7 at k = t / 32 (mod 8), so each half-wave
+// reads 512 contiguous bytes of one B_g row.  A's rows are staged as FP32 in
+// LDS per 256-k chunk; the 8 k-phases are summed through LDS at the end.
+template <int NR, bool OUT_BF16>
 __global__ void __launch_bounds__(256)
-    vbg_grouped_db_small(const int* small, int n_small, const int* msize, const int64_t* rowoff,
-                         int K, int N, const unsigned short* A, const unsigned short* dC, void* dB) {
-  const int64_t n8 = N / 8;
+    vbg_grouped_fwd_small(const int* groups, int n_groups, const int* msize, const int64_t* rowoff,
+                          int K, int N, const u16* A, const u16* B, void* C) {
+  __shared__ float As[256][NR];
+  __shared__ float red[8][256];
+  const int t = threadIdx.x, cg = t & 31, kp = t >> 5;
+  const int c = blockIdx.x * 256 + 8 * cg;
+  for (int j = blockIdx.y; j < n_groups; j += gridDim.y) {
+    const int g = groups[j];
+    const int mg = msize[g];
+    const int64_t off = rowoff[g];
+    const u16* Bg = B + (int64_t)g * K * N;
+    float acc[NR][8];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) acc[r][e] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += 256) {
+      __syncthreads();  // the previous chunk's reads of As are done
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int k = k0 + t;
+        As[t][r] = (r < mg && k < K) ? bf16_to_f32(A[(off + r) * K + k]) : 0.0f;
+      }
+      __syncthreads();
+      const int kn = min(256, K - k0);
+      // rows of B_g in flight per lane: fewer for wide buckets (registers)
+      constexpr int UNR = NR >= 16 ? 2 : NR >= 8 ? 4 : 8;
+      if (c < N) {
+#pragma unroll UNR
+        for (int kk = kp; kk < kn; kk += 8) {
+          float b[8];
+          bf16x8_to_f32(*(const uint4*)(Bg + (int64_t)(k0 + kk) * N + c), b);
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const float a = As[kk][r];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[r][e] = fmaf(a, b[e], acc[r][e]);
+          }
+        }
+      }
+    }
+    // sum the 8 k-phases: red[kp][column] per row, then thread t owns column t
+    // (a counted loop over all NR rows, so acc is indexed statically; rows
+    // past M_g are not stored)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < 8; ++e) red[kp][8 * cg + e] = acc[r][e];
+      __syncthreads();
+      const int col = blockIdx.x * 256 + t;
+      if (col < N && r < mg) {
+        float v = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v += red[q][t];
+        const int64_t o = (off + r) * N + col;
+        if (OUT_BF16) {
+          const __bf16 h = (__bf16)v;
+          ((u16*)C)[o] = __builtin_bit_cast(unsigned short, h);
+        } else {
+          ((float*)C)[o] = v;
+        }
+      }
+    }
+  }
+}
+
+// dA of the small groups: dA[off_g + r][k] = sum_c dC[off_g + r][c] *
+// B_g[k][c], r < M_g <= NR.  The cost is streaming B_g once.  Block
+// (blockIdx.x, group blockIdx.y): 8 * RPL rows of B_g; a half-wave takes one
+// row at a time, 8 consecutive columns per lane (512 contiguous bytes per row
+// per step), each lane keeping the partial sums of its RPL rows.  dC's rows
+// are staged as FP32 in dynamic LDS, cw columns at a time (all of N when it
+// fits in 48 KB, so the block's loads run without barriers between them); the
+// 32 lanes of a row are summed by xor-shuffles.
+template <int NR>
+struct DaSmall {
+  static constexpr int RPL = 4;                // rows per lane (8 RPL rows per block)
+  static constexpr int ROWS = 8 * RPL;         // rows of B_g per block
+  static constexpr int LDS_BYTES = 48 * 1024;
+  static int cw(int N) {  // staged columns: a multiple of 256, or all of N
+    const int cap = LDS_BYTES / (4 * NR) / 256 * 256;
+    return N <= cap ? N : cap;
+  }
+};
+
+template <int NR, bool OUT_BF16>
+__global__ void __launch_bounds__(256)
+    vbg_grouped_da_small(const int* groups, int n_groups, const int* msize, const int64_t* rowoff,
+                         int K, int N, int cw, const u16* B, const u16* dC, void* dA) {
+  constexpr int RPL = DaSmall<NR>::RPL, ROWS = DaSmall<NR>::ROWS;
+  extern __shared__ float Ds[];  // [NR][cw]
+  const int t = threadIdx.x, l = t & 63, w = t >> 6;
+  const int cq = 8 * (l & 31);
+  int krow[RPL];
+#pragma unroll
+  for (int i = 0; i < RPL; ++i) krow[i] = blockIdx.x * ROWS + 2 * RPL * w + 2 * i + (l >> 5);
+  for (int j = blockIdx.y; j < n_groups; j += gridDim.y) {
+    const int g = groups[j];
+    const int mg = msize[g];
+    const int64_t off = rowoff[g];
+    const u16* Bg = B + (int64_t)g * K * N;
+    float acc[RPL][NR];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i)
+#pragma unroll
+      for (int r = 0; r < NR; ++r) acc[i][r] = 0.0f;
+    for (int c0 = 0; c0 < N; c0 += cw) {
+      __syncthreads();  // the previous chunk's reads of Ds are done
+      for (int i = t; i < NR * cw; i += 256) {
+        const int r = i / cw, c = c0 + i % cw;
+        Ds[i] = (r < mg && c < N) ? bf16_to_f32(dC[(off + r) * N + c]) : 0.0f;
+      }
+      __syncthreads();
+      const int cn = min(cw, N - c0);
+      for (int cc = cq; cc < cn; cc += 256) {
+        float d[NR][8];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const float4 d0 = *(const float4*)&Ds[r * cw + cc], d1 = *(const float4*)&Ds[r * cw + cc + 4];
+          d[r][0] = d0.x; d[r][1] = d0.y; d[r][2] = d0.z; d[r][3] = d0.w;
+          d[r][4] = d1.x; d[r][5] = d1.y; d[r][6] = d1.z; d[r][7] = d1.w;
+        }
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+          float b[8];
+          bf16x8_to_f32(*(const uint4*)(Bg + (int64_t)min(krow[i], K - 1) * N + c0 + cc), b);
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            float s = acc[i][r];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s = fmaf(b[e], d[r][e], s);
+            acc[i][r] = s;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RPL; ++i)
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        float v = acc[i][r];
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 8);
+        v += __shfl_xor(v, 16);
+        if ((l & 31) == 0 && r < mg && krow[i] < K) {
+          const int64_t o = (off + r) * K + krow[i];
+          if (OUT_BF16) {
+            const __bf16 h = (__bf16)v;
+            ((u16*)dA)[o] = __builtin_bit_cast(unsigned short, h);
+          } else {
+            ((float*)dA)[o] = v;
+          }
+        }
+      }
+  }
+}
+
+// dB of the small groups: dB_g[k][n] = sum_{r < M_g} A[off_g + r][k] *
+// dC[off_g + r][n], M_g <= NR (M_g = 0 writes zeros).  The cost is writing
+// dB_g (K x N).  Thread: 4 consecutive k x 8 consecutive n (each dC load
+// serves 4 outputs rows); consecutive threads take consecutive n, so every
+// row's stores are contiguous.  BF16 products are exact in FP32, so only the
+// row-order additions round.
+template <int NR, bool OUT_BF16>
+__global__ void __launch_bounds__(256)
+    vbg_grouped_db_small(const int* groups, int n_groups, const int* msize, const int64_t* rowoff,
+                         int K, int N, const u16* A, const u16* dC, void* dB) {
+  const int64_t n8 = N / 8, k4 = (K + 3) / 4;
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (int64_t)K * n8) return;
-  const int k = (int)(idx / n8), n0 = (int)(idx - k * n8) * 8;
-  for (int j = blockIdx.y; j < n_small; j += gridDim.y) {  // groups beyond gridDim.y loop
-  const int g = small[j];
-  const int64_t off = rowoff[g];
-  const int mg = msize[g];
-  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  for (int r = 0; r < mg; ++r) {
-    const float av = bf16_to_f32(A[(off + r) * K + k]);
-    const uint4 raw = *(const uint4*)(dC + (off + r) * N + n0);
-    const unsigned w[4] = {raw.x, raw.y, raw.z, raw.w};
+  if (idx >= k4 * n8) return;
+  const int kq = (int)(idx / n8), n0 = (int)(idx - kq * n8) * 8;
+  const int kb = 4 * kq;
+  for (int j = blockIdx.y; j < n_groups; j += gridDim.y) {
+    const int g = groups[j];
+    const int mg = msize[g];
+    const int64_t off = rowoff[g];
+    float acc[4][8];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      acc[2 * e] = fmaf(av, __builtin_bit_cast(float, w[e] << 16), acc[2 * e]);
-      acc[2 * e + 1] = fmaf(av, __builtin_bit_cast(float, w[e] & 0xffff0000u), acc[2 * e + 1]);
-    }
-  }
-  const int64_t o = (int64_t)g * K * N + (int64_t)k * N + n0;
-  if (OUT_BF16) {
-    unsigned w[4];
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const __bf16 lo = (__bf16)acc[2 * e], hi = (__bf16)acc[2 * e + 1];
-      w[e] = (unsigned)__builtin_bit_cast(unsigned short, lo) |
-             ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
+      for (int e = 0; e < 8; ++e) acc[q][e] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r >= mg) break;
+      float d[8];
+      bf16x8_to_f32(*(const uint4*)(dC + (off + r) * N + n0), d);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float a = kb + q < K ? bf16_to_f32(A[(off + r) * K + kb + q]) : 0.0f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[q][e] = fmaf(a, d[e], acc[q][e]);
+      }
     }
-    *(uint4*)((unsigned short*)dB + o) = uint4{w[0], w[1], w[2], w[3]};
-  } else {
-    float4* q = (float4*)((float*)dB + o);
-    q[0] = float4{acc[0], acc[1], acc[2], acc[3]};
-    q[1] = float4{acc[4], acc[5], acc[6], acc[7]};
-  }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (kb + q >= K) break;
+      const int64_t o = (int64_t)g * K * N + (int64_t)(kb + q) * N + n0;
+      if (OUT_BF16) {
+        *(uint4*)((u16*)dB + o) = f32x8_to_bf16(acc[q]);
+      } else {
+        float4* p = (float4*)((float*)dB + o);
+        p[0] = float4{acc[q][0], acc[q][1], acc[q][2], acc[q][3]};
+        p[1] = float4{acc[q][4], acc[q][5], acc[q][6], acc[q][7]};
+      }
+    }
   }
 }
 
@@ -118,14 +321,50 @@ struct vbg_grouped_plan {
   int64_t* rowoff = nullptr;
   void** ptrs[3][3] = {};  // [pass][A', B', C']
   int* msize = nullptr;    // [G] rows per group
-  int* big = nullptr;      // groups whose dB runs on the GEMM plan
-  int* small = nullptr;    // groups whose dB runs on vbg_grouped_db_small
-  int n_big = 0, n_small = 0;
+  int* big = nullptr;      // groups of more than kSmallRows rows: the dA / dB GEMM plans
+  int* fwd_gemm = nullptr; // the forward GEMM plan's groups: big ones and thin buckets' ones
+  int* small = nullptr;    // groups of 1..kSmallRows rows, sorted by row bucket
+  int* zero = nullptr;     // groups of 0 rows (dB_g = 0)
+  int n_big = 0, n_fwd_gemm = 0, n_zero = 0;
+  int bucket_off[kBuckets + 1] = {};  // small[bucket_off[b] .. bucket_off[b+1]) have M_g <= 2^b
+  bool fwd_small[kBuckets] = {};      // the forward runs bucket b on vbg_grouped_fwd_small
 };
 
 static int gfail(int code, const std::string& m) {
   g_gerr = m;
   return code;
+}
+
+// One small-group kernel launch per row bucket that holds groups.
+template <bool OB>
+static void launch_small(int pass, int b, const int* groups, int n, const vbg_grouped_plan* p,
+                         const void* a, const void* bb, const void* c, void* x, hipStream_t s) {
+  const int K = p->K, N = p->N;
+  const unsigned gy = (unsigned)std::min(n, 65535);
+  const u16 *A = (const u16*)a, *B = (const u16*)bb, *dC = (const u16*)c;
+#define VBG_SMALL(NR)                                                                                   \
+  do {                                                                                                  \
+    if (pass == PASS_FWD)                                                                               \
+      hipLaunchKernelGGL((vbg_grouped_fwd_small<NR, OB>), dim3((N + 255) / 256, gy), dim3(256), 0, s,  \
+                         groups, n, p->msize, p->rowoff, K, N, A, B, x);                                \
+    else if (pass == PASS_DA)                                                                           \
+      hipLaunchKernelGGL((vbg_grouped_da_small<NR, OB>),                                                \
+                         dim3((K + DaSmall<NR>::ROWS - 1) / DaSmall<NR>::ROWS, gy), dim3(256),          \
+                         (size_t)4 * NR * DaSmall<NR>::cw(N), s, groups, n, p->msize, p->rowoff, K, N,  \
+                         DaSmall<NR>::cw(N), B, dC, x);                                                 \
+    else                                                                                                \
+      hipLaunchKernelGGL((vbg_grouped_db_small<NR, OB>),                                                \
+                         dim3((unsigned)(((int64_t)(K + 3) / 4 * (N / 8) + 255) / 256), gy), dim3(256), \
+                         0, s, groups, n, p->msize, p->rowoff, K, N, A, dC, x);                         \
+  } while (0)
+  switch (b) {
+    case 0: VBG_SMALL(1); break;
+    case 1: VBG_SMALL(2); break;
+    case 2: VBG_SMALL(4); break;
+    case 3: VBG_SMALL(8); break;
+    default: VBG_SMALL(16); break;
+  }
+#undef VBG_SMALL
 }
 
 extern "C" {
@@ -163,49 +402,84 @@ int vbg_grouped_plan_create(vbg_grouped_plan** plan, int G, const int* M_host, i
     delete p;
     return gfail(VBG_ERR_HIP, hipGetErrorString(e));
   }
-  std::vector<int> m(M_host, M_host + G), kk(G, K), nn(G, N);
-  std::vector<int> lN(G, N), lK(G, K), lM(G);
-  for (int g = 0; g < G; ++g) lM[g] = std::max(1, M_host[g]);
-  std::vector<int> big, small, bigN, bigK, bigM;
+  std::vector<int> big, zero, bigN, bigK, bigM;
+  std::vector<std::vector<int>> bucket(kBuckets);
   for (int g = 0; g < G; ++g) {
-    if (M_host[g] > kSmallRows) {
+    const int mg = M_host[g];
+    if (mg > kSmallRows) {
       big.push_back(g);
       bigN.push_back(N);
       bigK.push_back(K);
-      bigM.push_back(M_host[g]);
+      bigM.push_back(mg);
+    } else if (mg == 0) {
+      zero.push_back(g);
     } else {
-      small.push_back(g);
+      int b = 0;
+      while (bucket_rows(b) < mg) ++b;
+      bucket[b].push_back(g);
     }
   }
+  std::vector<int> small;
+  p->bucket_off[0] = 0;
+  for (int b = 0; b < kBuckets; ++b) {
+    small.insert(small.end(), bucket[b].begin(), bucket[b].end());
+    p->bucket_off[b + 1] = (int)small.size();
+  }
+  // The forward's streaming kernel has one block per (group, 256 columns) and
+  // nothing else in flight: a bucket with fewer blocks than kFwdMinBlocks (one
+  // per CU) would be one long latency-bound launch, so its groups join the
+  // forward GEMM plan instead (measured on config 4: a lone 5-row group took
+  // 173 us on 6 blocks).
+  const int kFwdMinBlocks = 256;
+  std::vector<int> fg = big, fgN = bigN, fgK = bigK, fgM = bigM;
+  for (int b = 0; b < kBuckets; ++b) {
+    p->fwd_small[b] = (int64_t)bucket[b].size() * ((N + 255) / 256) >= kFwdMinBlocks;
+    if (p->fwd_small[b]) continue;
+    for (int g : bucket[b]) {
+      fg.push_back(g);
+      fgN.push_back(N);
+      fgK.push_back(K);
+      fgM.push_back(M_host[g]);
+    }
+  }
+  // The GEMM plans cover the big groups (entry j = group big[j]):
   // forward: (m=N, n=M_g, k=K) NN, lda=N, ldb=K, ldc=N
   // dA:      (m=K, n=M_g, k=N) TN, lda=N, ldb=N, ldc=K
   // dB:      (m=N, n=K, k=M_g) NT, lda=N, ldb=K, ldc=N
   int rc = VBG_OK;
-  for (int o = 0; o < 2 && rc == VBG_OK; ++o) {
+  for (int o = 0; o < 2 && rc == VBG_OK && !fg.empty(); ++o) {
     const int prec = o ? VBG_PREC_BF16_OUT_BF16 : VBG_PREC_BF16;
-    rc = vbg_plan_create(&p->plans[PASS_FWD][o], G, nn.data(), m.data(), kk.data(), lN.data(),
-                         lK.data(), lN.data(), 'N', 'N', prec, opts);
-    if (rc == VBG_OK)
-      rc = vbg_plan_create(&p->plans[PASS_DA][o], G, kk.data(), m.data(), nn.data(), lN.data(),
-                           lN.data(), lK.data(), 'T', 'N', prec, opts);
-    if (rc == VBG_OK && !big.empty())
-      rc = vbg_plan_create(&p->plans[PASS_DB][o], (int)bigN.size(), bigN.data(), bigK.data(),
-                           bigM.data(), bigN.data(), bigK.data(), bigN.data(), 'N', 'T', prec, opts);
+    rc = vbg_plan_create(&p->plans[PASS_FWD][o], (int)fg.size(), fgN.data(), fgM.data(), fgK.data(), fgN.data(),
+                         fgK.data(), fgN.data(), 'N', 'N', prec, opts);
+  }
+  if (rc == VBG_OK && !big.empty()) {
+    const int nb = (int)big.size();
+    for (int o = 0; o < 2 && rc == VBG_OK; ++o) {
+      const int prec = o ? VBG_PREC_BF16_OUT_BF16 : VBG_PREC_BF16;
+      if (rc == VBG_OK)
+        rc = vbg_plan_create(&p->plans[PASS_DA][o], nb, bigK.data(), bigM.data(), bigN.data(), bigN.data(),
+                             bigN.data(), bigK.data(), 'T', 'N', prec, opts);
+      if (rc == VBG_OK)
+        rc = vbg_plan_create(&p->plans[PASS_DB][o], nb, bigN.data(), bigK.data(), bigM.data(), bigN.data(),
+                             bigK.data(), bigN.data(), 'N', 'T', prec, opts);
+    }
   }
   if (rc != VBG_OK) {
     g_gerr = vbg_last_error();
     vbg_grouped_plan_destroy(p);
     return rc;
   }
-  // device layout: rowoff[G] | 9 pointer arrays [G] | msize[G] | big[G] | small[G]
+  // device layout: rowoff[G] | 9 pointer arrays [G] | msize[G] | big[G] | small[G] | zero[G] | fwd_gemm[G]
   const size_t ints_off = sizeof(int64_t) * G + 9 * sizeof(void*) * (size_t)G;
-  const size_t bytes = ints_off + 3 * sizeof(int) * (size_t)G;
+  const size_t bytes = ints_off + 5 * sizeof(int) * (size_t)G;
   std::vector<char> host(bytes, 0);
   std::memcpy(host.data(), off.data(), sizeof(int64_t) * G);
   int* hints = (int*)(host.data() + ints_off);
   std::memcpy(hints, M_host, sizeof(int) * G);
   if (!big.empty()) std::memcpy(hints + G, big.data(), sizeof(int) * big.size());
   if (!small.empty()) std::memcpy(hints + 2 * G, small.data(), sizeof(int) * small.size());
+  if (!zero.empty()) std::memcpy(hints + 3 * G, zero.data(), sizeof(int) * zero.size());
+  if (!fg.empty()) std::memcpy(hints + 4 * G, fg.data(), sizeof(int) * fg.size());
   e = hipMalloc(&p->dbuf, bytes);
   if (e == hipSuccess) e = hipMemcpy(p->dbuf, host.data(), bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -220,8 +494,11 @@ int vbg_grouped_plan_create(vbg_grouped_plan** plan, int G, const int* M_host, i
   p->msize = dints;
   p->big = dints + G;
   p->small = dints + 2 * G;
+  p->zero = dints + 3 * G;
+  p->fwd_gemm = dints + 4 * G;
   p->n_big = (int)big.size();
-  p->n_small = (int)small.size();
+  p->n_fwd_gemm = (int)fg.size();
+  p->n_zero = (int)zero.size();
   *plan = p;
   return VBG_OK;
 }
@@ -230,13 +507,14 @@ static bool aligned16(const void* q) { return ((uintptr_t)q & 15) == 0; }
 
 static int run_pass(vbg_grouped_plan* p, int pass, const void* a, const void* b, const void* c,
                     void* x, int c_es, int x_es, int out_bf16, hipStream_t s) {
-  // dB: the GEMM plan covers the big groups only (entry j = group big[j])
-  const int n = pass == PASS_DB ? p->n_big : p->G;
-  const int* gidx = pass == PASS_DB ? p->big : nullptr;
+  // the GEMM plans cover the big groups, and for the forward the groups of
+  // thin buckets too (entry j = group gidx[j])
+  const int n = pass == PASS_FWD ? p->n_fwd_gemm : p->n_big;
+  const int* gidx = pass == PASS_FWD ? p->fwd_gemm : p->big;
   if (n > 0) {
-    hipLaunchKernelGGL(vbg_grouped_ptrs, dim3((n + 255) / 256), dim3(256), 0, s, pass, n, gidx, p->K,
-                       p->N, p->rowoff, (const char*)a, (const char*)b, (const char*)c, (char*)x,
-                       c_es, x_es, (const void**)p->ptrs[pass][0], (const void**)p->ptrs[pass][1],
+    hipLaunchKernelGGL(vbg_grouped_ptrs, dim3((n + 255) / 256), dim3(256), 0, s, pass, n,
+                       gidx, p->K, p->N, p->rowoff, (const char*)a, (const char*)b, (const char*)c,
+                       (char*)x, c_es, x_es, (const void**)p->ptrs[pass][0], (const void**)p->ptrs[pass][1],
                        p->ptrs[pass][2]);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return gfail(VBG_ERR_HIP, hipGetErrorString(e));
@@ -246,20 +524,20 @@ static int run_pass(vbg_grouped_plan* p, int pass, const void* a, const void* b,
                                       (float* const*)p->ptrs[pass][2], s);
     if (rc != VBG_OK) return rc;
   }
-  if (pass == PASS_DB && p->n_small > 0) {
-    const int64_t threads = (int64_t)p->K * (p->N / 8);
-    const dim3 grid((unsigned)((threads + 255) / 256), (unsigned)std::min(p->n_small, 65535));
-    if (out_bf16)
-      hipLaunchKernelGGL(vbg_grouped_db_small<true>, grid, dim3(256), 0, s, p->small, p->n_small,
-                         p->msize, p->rowoff, p->K, p->N, (const unsigned short*)a,
-                         (const unsigned short*)c, x);
-    else
-      hipLaunchKernelGGL(vbg_grouped_db_small<false>, grid, dim3(256), 0, s, p->small, p->n_small,
-                         p->msize, p->rowoff, p->K, p->N, (const unsigned short*)a,
-                         (const unsigned short*)c, x);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return gfail(VBG_ERR_HIP, hipGetErrorString(e));
+  // the small groups, one launch per row bucket; dB also zeroes the empty groups
+  for (int bk = 0; bk < kBuckets; ++bk) {
+    const int nb = p->bucket_off[bk + 1] - p->bucket_off[bk];
+    if (nb == 0 || (pass == PASS_FWD && !p->fwd_small[bk])) continue;
+    const int* groups = p->small + p->bucket_off[bk];
+    if (out_bf16) launch_small<true>(pass, bk, groups, nb, p, a, b, c, x, s);
+    else launch_small<false>(pass, bk, groups, nb, p, a, b, c, x, s);
   }
+  if (pass == PASS_DB && p->n_zero > 0) {
+    if (out_bf16) launch_small<true>(pass, 0, p->zero, p->n_zero, p, a, b, c, x, s);
+    else launch_small<false>(pass, 0, p->zero, p->n_zero, p, a, b, c, x, s);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return gfail(VBG_ERR_HIP, hipGetErrorString(e));
   return VBG_OK;
 }
 

@@ -184,3 +184,38 @@ def test_grouped_forced_split_k(bg, gpu):
     assert rel(C16.float().cpu().numpy(), C) <= 4e-3
     assert rel(dA.cpu().numpy(), rA) <= 1e-5
     assert rel(dB.cpu().numpy(), rB) <= 1e-5
+
+
+def test_grouped_small_groups_every_bucket(bg, gpu):
+    """Many groups of 0..16 rows: every row bucket (1, 2, 4, 8, 16) of the
+    small-group kernels holds enough groups that the forward streams them too
+    (N = 1280: 5 column blocks per group), with a few big groups on the GEMM
+    plans beside them.  C, dA, dB against FP64, FP32 and BF16 outputs."""
+    import torch
+    from batchgemm_amd.grouped import GroupedPlan
+    say("[grouped] small groups in every bucket")
+    rng = np.random.default_rng(9)
+    m_sizes = [int(x) for x in rng.integers(1, 17, size=300)] + [0, 0, 40, 130, 0]
+    rng.shuffle(m_sizes)
+    K, N = 72, 1280
+    A, B, dC = make(m_sizes, K, N, seed=9)
+    p = GroupedPlan(m_sizes, K, N)
+    C32 = p.forward(A, B, out_dtype=torch.float32)
+    C16 = p.forward(A, B, out_dtype=torch.bfloat16)
+    dA, dB = p.backward(A, B, dC, grad_dtype=torch.float32)
+    dA16, dB16 = p.backward(A, B, dC, grad_dtype=torch.bfloat16)
+    torch.cuda.synchronize()
+    C, rA, rB = ref64(A, B, dC, m_sizes)
+    assert rel(C32.cpu().numpy(), C) <= 1e-5
+    assert rel(C16.float().cpu().numpy(), C) <= 4e-3
+    assert rel(dA.cpu().numpy(), rA) <= 1e-5
+    assert rel(dB.cpu().numpy(), rB) <= 1e-5
+    assert rel(dA16.float().cpu().numpy(), rA) <= 4e-3
+    assert rel(dB16.float().cpu().numpy(), rB) <= 4e-3
+    off = np.concatenate([[0], np.cumsum(m_sizes)])
+    for g in np.where(np.asarray(m_sizes) == 0)[0]:  # empty groups: zero dB_g
+        assert float(dB[g].abs().max()) == 0.0
+    for g in range(len(m_sizes)):  # per-group check catches a bucket that is off
+        s, e = int(off[g]), int(off[g + 1])
+        if e > s:
+            assert rel(C32[s:e].cpu().numpy(), C[s:e]) <= 1e-5, (g, m_sizes[g])
