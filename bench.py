@@ -304,7 +304,7 @@ def main():
         traffic_probe(bg, vb, M, N, K, alpha, beta, args)
         return
     t0 = time.perf_counter()
-    plan = bg.Plan(M, N, K, prec=args.prec, policy=args.policy, timing_slots=args.steps)
+    plan = bg.Plan(M, N, K, prec=args.prec, policy=args.policy)
     plan_ms = (time.perf_counter() - t0) * 1e3
     log(f"[bench] plan: {plan.n_items} items, {plan.n_launches} launches, grid {plan.grid}, "
         f"{plan_ms:.1f} ms")
@@ -314,12 +314,15 @@ def main():
     value = flops / (ms * 1e-3) / 1e9
     log(f"[bench] {ms:.3f} ms/step  {value:.1f} GFLOP/s")
 
-    # Per-kernel durations of the timed steps (HIP events recorded by the plan
-    # around each of its kernels, on the launch stream) and each kernel's
-    # algorithmic flops (2*rows*cols*k of its items, padding excluded).
-    kt = plan.kernels()
-    kms = plan.kernel_times()
-    iflops = plan.item_flops()
+    # Per-kernel durations (HIP events recorded by a timing plan of the same
+    # batch around each of its kernels, on the launch stream; a timing plan
+    # runs its launches one after another) and each kernel's algorithmic flops
+    # (2*rows*cols*k of its items, padding excluded).
+    tplan = bg.Plan(M, N, K, prec=args.prec, policy=args.policy, timing_slots=args.steps)
+    time_launches(lambda: tplan.run(vb.dA, vb.dB, vb.dC, alpha, beta), args.steps, args.warmup)
+    kt = tplan.kernels()
+    kms = tplan.kernel_times()
+    iflops = tplan.item_flops()
     rows = []
     for (kind, tile, first, cnt, grid), t in zip(kt, kms):
         f = float(iflops[first:first + cnt].sum()) if kind in (1, 2, 3, 4) else 0.0
