@@ -34,10 +34,15 @@ __device__ __forceinline__ int bswz(int row, int byte) {
 // loads (4 mn each, one per k), a 16-bit transpose in registers (one byte
 // permute per output word), then 4 LDS writes of KB k each (16 or 8 bytes).
 // Lanes 0..31 of a wave cover 128 consecutive mn at one k: 256-byte loads.
-// Buffer loads: a lane's byte offset is fixed once per work item and the
-// k-tile position is a wave-uniform scalar offset.  Rows past MN are clamped
-// (they only feed outputs that are never stored); k past K is zero; a vector
-// that straddles MN or K falls back to 2-byte loads.
+//   Every k-tile issues the same vector loads (no per-lane or per-tile
+// branches around them), so the k-loop's loads in flight are the same on
+// every path and the compiler can wait for exactly the oldest tile: a lane's
+// byte offset is fixed once per work item and the k-tile position is a
+// wave-uniform scalar offset.  Rows past MN are clamped (to MN - 1 for KC,
+// to a 4-aligned row for MC); they only feed outputs that are never stored.
+// Reads past the operand return 0 (buffer range); k >= K, which can lie
+// inside the operand's allocation, is zeroed by selects when the tile that
+// straddles K is stored to LDS.
 template <int ROWS, int BK, int PITCH, bool KC>
 struct Bf16Loader {
   static constexpr int KB = ROWS * BK / (4 * VBG_THREADS);  // MC: k per lane
@@ -48,10 +53,11 @@ struct Bf16Loader {
   struct Regs {
     v4u kc[KC ? NV : 1];
     v2u_t mc[KC ? 1 : KB];
+    int k0, kend;  // the k-tile these registers hold; k >= kend reads as zero
   };
   __amdgpu_buffer_rsrc_t rsrc;
   int voff[NV];
-  int ld2, MN, mn0;
+  int ld2;
 
   // MC lane map: each 8-lane group takes 4 consecutive mn-blocks (t = lane %
   // 4) at 2 consecutive k-groups (lane / 4 % 2).  Its 16-byte LDS writes of
@@ -72,27 +78,21 @@ struct Bf16Loader {
     }
   }
 
-  __device__ __forceinline__ void init(const u16* p, int ld, int mn0_, int MN_, int K) {
-    rsrc = vbg_rsrc(p, (uint32_t)ld * (uint32_t)(KC ? MN_ : K) * 2u);
+  __device__ __forceinline__ void init(const u16* p, int ld, int mn0, int MN, int K) {
+    rsrc = vbg_rsrc(p, (uint32_t)ld * (uint32_t)(KC ? MN : K) * 2u);
     ld2 = ld * 2;
-    MN = MN_;
-    mn0 = mn0_;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       int row, kk;
       coords(v, row, kk);
       if (KC) {
-        const int mnc = min(mn0 + row, MN - 1);
-        voff[v] = (mnc * ld + kk) * 2;
+        voff[v] = (min(mn0 + row, MN - 1) * ld + kk) * 2;
       } else {
-        voff[v] = (min(mn0 + row, MN - 1) + kk * ld) * 2;
+        voff[v] = (min(mn0 + row, (MN - 1) & ~3) + kk * ld) * 2;
       }
     }
   }
 
-  __device__ __forceinline__ u32 ld16(int off, int soff) const {
-    return __builtin_amdgcn_raw_buffer_load_b16(rsrc, off, soff, 0);
-  }
   __device__ __forceinline__ v4u ld128(int off, int soff) const {
     return __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, soff, 0));
   }
@@ -100,80 +100,43 @@ struct Bf16Loader {
     return __builtin_bit_cast(v2u_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, soff, 0));
   }
 
-  // A whole k-tile inside the operand (k0 + BK <= K and, for MC, every row
-  // of the tile below MN) loads with straight-line code; the decision is
-  // wave-uniform, so the k-loop stays one basic block the scheduler can
-  // interleave with the LDS reads and MFMAs.  Other tiles take the per-lane
-  // edge path.
-  __device__ __forceinline__ void fetch(Regs& r, int k0, int K) const {
-    if (k0 + BK <= K && (KC || mn0 + ROWS <= MN)) {
-      if constexpr (KC) {
+  __device__ __forceinline__ void fetch(Regs& r, int k0, int kend) const {
+    r.k0 = k0;
+    r.kend = kend;
+    if constexpr (KC) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) r.kc[v] = ld128(voff[v], k0 * 2);
-      } else {
-#pragma unroll
-        for (int j = 0; j < KB; ++j) r.mc[j] = ld64(voff[0], (k0 + j) * ld2);
-      }
+      for (int v = 0; v < NV; ++v) r.kc[v] = ld128(voff[v], k0 * 2);
     } else {
-      fetch_edge(r, k0, K);
+#pragma unroll
+      for (int j = 0; j < KB; ++j) r.mc[j] = ld64(voff[0], (k0 + j) * ld2);
     }
   }
 
-  __device__ __forceinline__ void fetch_edge(Regs& r, int k0, int K) const {
+  __device__ __forceinline__ void store(Regs& r, char* plane) const {
+    const int K = r.kend;
+    const bool tail = r.k0 + BK > K;  // wave-uniform: the tile straddles the k end
     if constexpr (KC) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         int row, kk;
         coords(v, row, kk);
-        const int k = k0 + kk;
-        if (k + 8 <= K) {
-          r.kc[v] = ld128(voff[v], k0 * 2);
-        } else {
-          u32 w[4];
+        v4u x = r.kc[v];
+        if (tail) {
+          const int lim = K - (r.k0 + kk);  // elements of this vector below K
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const u32 lo = k + 2 * e < K ? ld16(voff[v] + 4 * e, k0 * 2) : 0u;
-            const u32 hi = k + 2 * e + 1 < K ? ld16(voff[v] + 4 * e + 2, k0 * 2) : 0u;
-            w[e] = lo | (hi << 16);
-          }
-          r.kc[v] = v4u{w[0], w[1], w[2], w[3]};
+          for (int e = 0; e < 4; ++e)
+            x[e] &= (2 * e < lim ? 0xffffu : 0u) | (2 * e + 1 < lim ? 0xffff0000u : 0u);
         }
+        *(v4u*)(plane + bswz<PITCH>(row, 2 * kk)) = x;
       }
     } else {
       int row, kk;
       coords(0, row, kk);
-      const int mn = mn0 + row, k = k0 + kk;
-      const bool whole = mn + 3 < MN;
+      if (tail) {
 #pragma unroll
-      for (int j = 0; j < KB; ++j) {
-        const int soff = (k0 + j) * ld2;
-        v2u_t x;
-        if (whole) {
-          x = ld64(voff[0], soff);
-        } else {  // straddles MN: element by element, clamped
-          const int c0 = min(mn, MN - 1);
-          u32 e[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) e[q] = ld16(voff[0] + 2 * (min(mn + q, MN - 1) - c0), soff);
-          x = v2u_t{e[0] | (e[1] << 16), e[2] | (e[3] << 16)};
-        }
-        if (k + j >= K) x = v2u_t{0u, 0u};
-        r.mc[j] = x;
+        for (int j = 0; j < KB; ++j)
+          if (r.k0 + kk + j >= K) r.mc[j] = v2u_t{0u, 0u};
       }
-    }
-  }
-
-  __device__ __forceinline__ static void store(const Regs& r, char* plane) {
-    if constexpr (KC) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        int row, kk;
-        coords(v, row, kk);
-        *(v4u*)(plane + bswz<PITCH>(row, 2 * kk)) = r.kc[v];
-      }
-    } else {
-      int row, kk;
-      coords(0, row, kk);
       // element q of the 4-mn vector at k + j is 16-bit half q % 2 of word q / 2
       auto pair = [&](int j, int q) -> u32 {
         const u32 a = r.mc[j][q >> 1], b = r.mc[j + 1][q >> 1];
@@ -228,12 +191,14 @@ __device__ __forceinline__ void store4_bf16(u16* C, int ldc, int row, int col, i
 
 // Register stages of the k-loop: 2 = tile kt + 2 loads while tile kt + 1 is
 // stored between the MFMA steps of tile kt; 1 = tile kt + 1 loads during tile
-// kt and is stored after it.  Measured on the grouped config (G = 128, K =
-// 2048, N = 1408), with the straight-line whole-tile fetch: 1 stage fwd 0.535
-// ms, bwd 1.140; 2 stages fwd 0.551, bwd 1.197 (the dense kernels lose 4-13%,
-// the HBM-bound skinny-group kernels do not move).
+// kt and is stored after it.  Two stages only pay once every path has the same
+// loads in flight (branch-free fetch, unconditional clamped loads): before
+// that the compiler drained all loads (vmcnt(0)) ahead of each store and the
+// second stage did nothing.  Measured on the grouped config (G = 128, K =
+// 2048, N = 1408), two runs each: 2 stages fwd 0.469 / 0.473 ms, bwd 1.014 /
+// 1.012; 1 stage fwd 0.478 / 0.478, bwd 1.027 / 1.026.
 #ifndef VBG_BF16_STAGES
-#define VBG_BF16_STAGES 1
+#define VBG_BF16_STAGES 2
 #endif
 
 template <int BM, int BN, bool TA, bool TB, bool OUT_BF16>
@@ -299,31 +264,32 @@ struct Bf16Tile {
       // the MFMA steps and the loads of tile kt + 2 are in flight.
       typename LA::Regs ra0, ra1;
       typename LB::Regs rb0, rb1;
+      // The loads and stores are unconditional (the k-tile index is clamped
+      // to the last tile; a surplus tile lands in the LDS buffer no wave reads
+      // again), so every path has the same loads in flight and the compiler
+      // can wait for tile kt + 1 alone (vmcnt(n)) instead of draining tile
+      // kt + 2's loads too (vmcnt(0)), which would undo the second stage.
       auto step = [&](int kt, typename LA::Regs& nxa, typename LB::Regs& nxb, typename LA::Regs& lda_,
                       typename LB::Regs& ldb_) {
         const int cur = kt & 1;
-        if (kt + 2 < nk) {
-          la.fetch(lda_, kb + (kt + 2) * BK, ke);
-          lb.fetch(ldb_, kb + (kt + 2) * BK, ke);
-        }
+        const int kf = min(kt + 2, nk - 1);
+        la.fetch(lda_, kb + kf * BK, ke);
+        lb.fetch(ldb_, kb + kf * BK, ke);
         char* other = lds + (cur ^ 1) * STAGE;
-        const bool more = kt + 1 < nk;
         compute(lds + cur * STAGE, [&](int s) {
-          if (!more) return;
-          if (s == 0) LA::store(nxa, other);
-          if (s == BK / 32) LB::store(nxb, other + PLANE_A);
+          if (s == 0) la.store(nxa, other);
+          if (s == BK / 32) lb.store(nxb, other + PLANE_A);
         });
         __syncthreads();
       };
       if (nk > 0) {
         la.fetch(ra0, kb, ke);
         lb.fetch(rb0, kb, ke);
-        LA::store(ra0, lds);
-        LB::store(rb0, lds + PLANE_A);
-      }
-      if (nk > 1) {
-        la.fetch(ra1, kb + BK, ke);
-        lb.fetch(rb1, kb + BK, ke);
+        la.store(ra0, lds);
+        lb.store(rb0, lds + PLANE_A);
+        const int k1 = min(1, nk - 1);
+        la.fetch(ra1, kb + k1 * BK, ke);
+        lb.fetch(rb1, kb + k1 * BK, ke);
       }
       __syncthreads();
       for (int kt = 0; kt < nk; kt += 2) {
@@ -336,8 +302,8 @@ struct Bf16Tile {
       if (nk > 0) {
         la.fetch(ra, kb, ke);
         lb.fetch(rb, kb, ke);
-        LA::store(ra, lds);
-        LB::store(rb, lds + PLANE_A);
+        la.store(ra, lds);
+        lb.store(rb, lds + PLANE_A);
       }
       __syncthreads();
       for (int kt = 0; kt < nk; ++kt) {
@@ -351,8 +317,8 @@ struct Bf16Tile {
         compute(lds + cur * STAGE, [](int) {});
         if (more) {
           char* nxt = lds + (cur ^ 1) * STAGE;
-          LA::store(ra, nxt);
-          LB::store(rb, nxt + PLANE_A);
+          la.store(ra, nxt);
+          lb.store(rb, nxt + PLANE_A);
         }
         __syncthreads();
       }
