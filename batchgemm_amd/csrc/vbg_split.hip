@@ -91,6 +91,12 @@ struct Mode;
 #ifndef VBG_SPLIT_FENCE
 #define VBG_SPLIT_FENCE 1
 #endif
+// The two-stage loop loads tile kt+2 only while one exists, so the compiler
+// drains every load (vmcnt(0)) before storing tile kt+1.  Making that load
+// unconditional (clamped to the last tile), which lets it wait for tile kt+1
+// alone, measured slower here (in-loop FP16 / FP8 GEMM kernels 3.38 -> 3.51 /
+// 3.48 ms; packed and bf16x3 two-stage variants lost too), unlike the BF16
+// tiles, so the split family keeps the conditional load.
 // Register stages on packed operands (no conversion between load and LDS).
 // Measured on the unbalanced B=2048 batch, GEMM kernel ms with 2 / 1 stages:
 // bf16x3 3.97 / 3.58, FP16 2.15 / 2.12, FP8 1.88 / 1.83.
