@@ -381,15 +381,20 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   // (1.26 ms) is pure cost.
   const bool packable = prec == VBG_PREC_BF16X3 || prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3;
   bool pack = packable && o.pack_operands > 0;
+  // The default also leaves a batch unpacked when its packed operands would
+  // exceed kPackMaxBytes of plan-owned memory (asking for packing explicitly
+  // has no limit beyond the 2 GiB per operand the kernels address).
   if (packable && o.pack_operands == 0 && prec != VBG_PREC_BF16X3) {
-    const double kPackMinAI = 64.0;
-    double flops = 0.0, bytes = 0.0;
+    const double kPackMinAI = 64.0, kPackMaxBytes = 8.0 * (1u << 30);
+    const double ebp = prec == VBG_PREC_FP16_SCALED ? 2.0 : 1.0;
+    double flops = 0.0, bytes = 0.0, packed = 0.0;
     for (int b = 0; b < batch; ++b) {
       if (M[b] == 0 || N[b] == 0 || K[b] == 0) continue;
       flops += 2.0 * M[b] * N[b] * K[b];
       bytes += 4.0 * ((double)M[b] * K[b] + (double)K[b] * N[b]);
+      packed += ebp * ((double)M[b] + N[b]) * (double)(cdiv(K[b], VBG_PACK_KALIGN) * VBG_PACK_KALIGN);
     }
-    pack = bytes > 0.0 && flops / bytes >= kPackMinAI;
+    pack = bytes > 0.0 && flops / bytes >= kPackMinAI && packed <= kPackMaxBytes;
   }
   const int fam = prec == VBG_PREC_FP32_SEQ ? FAM_SEQ
                   : prec == VBG_PREC_FP32   ? FAM_FP32

@@ -118,7 +118,8 @@ typedef struct vbg_opts {
                            GEMM kernels stream without converting; < 0 quantise on the
                            HBM -> LDS path instead; 0 = default (FP16 / FP8 packed when the
                            batch's arithmetic intensity, 2*sum(MNK) / 4*sum(MK+KN), is at
-                           least 64 flop/B; bf16x3 unpacked) */
+                           least 64 flop/B and the workspace stays within 8 GiB; bf16x3
+                           unpacked) */
 } vbg_opts;
 
 typedef struct vbg_plan vbg_plan;

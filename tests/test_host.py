@@ -224,6 +224,10 @@ def test_packed_operand_workspace(bg):
     assert bg.Plan(Ms, Ns, Ks, host_only=True, prec="fp8").packed_bytes == 0
     assert bg.Plan(Ms, Ns, Ks, host_only=True, prec="fp8", pack_operands=1).packed_bytes > 0
     assert bg.Plan(Ms, Ns, Ks, host_only=True, prec="fp32", pack_operands=1).packed_bytes == 0
+    # the default does not pack past 8 GiB of workspace (80 x 8192^3 FP8: 10 GiB), a request does
+    Mb = np.full(80, 8192, np.int32)
+    assert bg.Plan(Mb, Mb, Mb, host_only=True, prec="fp8").packed_bytes == 0
+    assert bg.Plan(Mb[:8], Mb[:8], Mb[:8], host_only=True, prec="fp8").packed_bytes == 8 * 2 * 8192 * 8192
 
 
 def test_driver_gflop_count_matches_reference_logs(bg):
