@@ -292,10 +292,18 @@ struct Bf16Tile {
         lb.fetch(rb1, kb + k1 * BK, ke);
       }
       __syncthreads();
-      for (int kt = 0; kt < nk; kt += 2) {
+      // The odd last step is peeled out of the two-step loop: with it inside
+      // (as a conditional second step), the exit path rejoined the loop header
+      // with the first step's loads still in flight into registers the next
+      // step reuses, and the compiler waited for every load (vmcnt(0)) at the
+      // top of each step.  Grouped backward (G = 128, K = 2048, N = 1408):
+      // 1.020 -> 1.012 ms.
+      int kt = 0;
+      for (; kt + 1 < nk; kt += 2) {
         step(kt, ra1, rb1, ra0, rb0);
-        if (kt + 1 < nk) step(kt + 1, ra0, rb0, ra1, rb1);
+        step(kt + 1, ra0, rb0, ra1, rb1);
       }
+      if (kt < nk) step(kt, ra1, rb1, ra0, rb0);
     } else {
       typename LA::Regs ra;
       typename LB::Regs rb;
