@@ -95,12 +95,23 @@ __device__ __forceinline__ uint4 f32x8_to_bf16(const float (&f)[8]) {
 // (blockIdx.x, group blockIdx.y): 256 consecutive columns over all of K; lane
 // t takes columns 8 (t % 32) .. + 7 at k = t / 32 (mod 8), so each half-wave
 // reads 512 contiguous bytes of one B_g row.  A's rows are staged as FP32 in
-// LDS per 256-k chunk; the 8 k-phases are summed through LDS at the end.
+// LDS per chunk of KC k (one batch of 16-byte loads per thread); the 8
+// k-phases are summed through LDS at the end.
+//   The B_g stream is pipelined: batch b+1 (UNR rows per lane) is loaded
+// before batch b is consumed, with the loads unconditional (clamped to the
+// chunk's last row) so every path has the same loads in flight; a block's
+// 1 MB (K = 2048, 256 columns) would otherwise take one round trip per batch
+// with nothing in flight between them.
 template <int NR, bool OUT_BF16>
 __global__ void __launch_bounds__(256)
     vbg_grouped_fwd_small(const int* groups, int n_groups, const int* msize, const int64_t* rowoff,
                           int K, int N, const u16* A, const u16* B, void* C) {
-  __shared__ float As[256][NR];
+  constexpr int KC = NR >= 16 ? 512 : 1024;  // k per staged chunk of A (at most 32 KB)
+  constexpr int UPR = KC / 8;                // 16-byte units per staged row
+  constexpr int NST = (NR * UPR + 255) / 256;
+  // rows of B_g per lane per batch: fewer for wide buckets (registers)
+  constexpr int UNR = NR >= 16 ? 2 : NR >= 8 ? 4 : 8;
+  __shared__ float As[KC][NR];
   __shared__ float red[8][256];
   const int t = threadIdx.x, cg = t & 31, kp = t >> 5;
   const int c = blockIdx.x * 256 + 8 * cg;
@@ -114,29 +125,72 @@ __global__ void __launch_bounds__(256)
     for (int r = 0; r < NR; ++r)
 #pragma unroll
       for (int e = 0; e < 8; ++e) acc[r][e] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += 256) {
+    for (int k0 = 0; k0 < K; k0 += KC) {
       __syncthreads();  // the previous chunk's reads of As are done
+      {
+        uint4 raw[NST];
+        unsigned keep[NST];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const int k = k0 + t;
-        As[t][r] = (r < mg && k < K) ? bf16_to_f32(A[(off + r) * K + k]) : 0.0f;
-      }
-      __syncthreads();
-      const int kn = min(256, K - k0);
-      // rows of B_g in flight per lane: fewer for wide buckets (registers)
-      constexpr int UNR = NR >= 16 ? 2 : NR >= 8 ? 4 : 8;
-      if (c < N) {
-#pragma unroll UNR
-        for (int kk = kp; kk < kn; kk += 8) {
-          float b[8];
-          bf16x8_to_f32(*(const uint4*)(Bg + (int64_t)(k0 + kk) * N + c), b);
+        for (int i = 0; i < NST; ++i) {
+          // unconditional load at a clamped address (a group here has M_g >= 1),
+          // masked in the second loop, so the batch's loads are all in flight
+          // at once (a conditional overwrite of the loaded registers would wait
+          // for each load before issuing the next)
+          const int u = t + 256 * i, uc = min(u, NR * UPR - 1), r = uc / UPR, k = k0 + 8 * (uc % UPR);
+          raw[i] = *(const uint4*)(A + (off + min(r, mg - 1)) * K + min(k, K - 8));
+          keep[i] = (u < NR * UPR && r < mg && k < K) ? 0xffffffffu : 0u;
+        }
 #pragma unroll
-          for (int r = 0; r < NR; ++r) {
-            const float a = As[kk][r];
+        for (int i = 0; i < NST; ++i) {
+          const int u = t + 256 * i, r = u / UPR, kk = 8 * (u % UPR);
+          if (u < NR * UPR) {
+            float v[8];
+            const uint4 x = raw[i];
+            bf16x8_to_f32(uint4{x.x & keep[i], x.y & keep[i], x.z & keep[i], x.w & keep[i]}, v);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) acc[r][e] = fmaf(a, b[e], acc[r][e]);
+            for (int e = 0; e < 8; ++e) As[kk + e][r] = v[e];
           }
         }
+      }
+      __syncthreads();
+      if (c < N) {
+        // this lane's rows of the chunk: kk = kp + 8 q, q < nq (K % 8 == 0)
+        const int nq = min(KC, K - k0) / 8;
+        const int nb = (nq + UNR - 1) / UNR;
+        const u16* bp = Bg + (int64_t)(k0 + kp) * N + c;
+        const int64_t step = 8 * (int64_t)N;
+        uint4 b0[UNR], b1[UNR];
+        auto load = [&](uint4 (&dst)[UNR], int b) {
+#pragma unroll
+          for (int i = 0; i < UNR; ++i) dst[i] = *(const uint4*)(bp + min(b * UNR + i, nq - 1) * step);
+        };
+        auto use = [&](const uint4 (&src)[UNR], int b) {
+#pragma unroll
+          for (int i = 0; i < UNR; ++i) {
+            const int q = b * UNR + i;
+            if (q < nq) {  // uniform over the block
+              float bv[8];
+              bf16x8_to_f32(src[i], bv);
+#pragma unroll
+              for (int r = 0; r < NR; ++r) {
+                const float a = As[kp + 8 * q][r];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) acc[r][e] = fmaf(a, bv[e], acc[r][e]);
+              }
+            }
+          }
+        };
+        // pairs of batches, the odd last one peeled: no path reaches the loop
+        // header with loads in flight into registers the next pair reloads
+        load(b0, 0);
+        int b = 0;
+        for (; b + 1 < nb; b += 2) {
+          load(b1, b + 1);
+          use(b0, b);
+          load(b0, min(b + 2, nb - 1));
+          use(b1, b + 1);
+        }
+        if (b < nb) use(b0, b);
       }
     }
     // sum the 8 k-phases: red[kp][column] per row, then thread t owns column t
@@ -194,7 +248,7 @@ __global__ void __launch_bounds__(256)
   const int cq = 8 * (l & 31);
   int krow[RPL];
 #pragma unroll
-  for (int i = 0; i < RPL; ++i) krow[i] = blockIdx.x * ROWS + 2 * RPL * w + 2 * i + (l >> 5);
+  for (int i = 0; i < RPL; ++i) krow[i] = min(blockIdx.x * ROWS + 2 * RPL * w + 2 * i + (l >> 5), K - 1);
   for (int j = blockIdx.y; j < n_groups; j += gridDim.y) {
     const int g = groups[j];
     const int mg = msize[g];
@@ -207,34 +261,82 @@ __global__ void __launch_bounds__(256)
       for (int r = 0; r < NR; ++r) acc[i][r] = 0.0f;
     for (int c0 = 0; c0 < N; c0 += cw) {
       __syncthreads();  // the previous chunk's reads of Ds are done
-      for (int i = t; i < NR * cw; i += 256) {
-        const int r = i / cw, c = c0 + i % cw;
-        Ds[i] = (r < mg && c < N) ? bf16_to_f32(dC[(off + r) * N + c]) : 0.0f;
-      }
-      __syncthreads();
-      const int cn = min(cw, N - c0);
-      for (int cc = cq; cc < cn; cc += 256) {
-        float d[NR][8];
+      // dC rows [r][c0, c0 + cw) as FP32, 16-byte loads (N and cw are
+      // multiples of 8), four per thread per batch
+      const int upr = cw / 8, nu = NR * upr;
+      for (int u0 = 0; u0 < nu; u0 += 4 * 256) {
+        uint4 raw[4];
+        unsigned keep[4];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-          const float4 d0 = *(const float4*)&Ds[r * cw + cc], d1 = *(const float4*)&Ds[r * cw + cc + 4];
-          d[r][0] = d0.x; d[r][1] = d0.y; d[r][2] = d0.z; d[r][3] = d0.w;
-          d[r][4] = d1.x; d[r][5] = d1.y; d[r][6] = d1.z; d[r][7] = d1.w;
+        for (int i = 0; i < 4; ++i) {
+          // unconditional load at a clamped address (a group here has M_g >= 1),
+          // masked in the second loop, so the batch's loads are all in flight
+          const int u = u0 + t + 256 * i, uc = min(u, nu - 1), r = uc / upr, c = c0 + 8 * (uc % upr);
+          raw[i] = *(const uint4*)(dC + (off + min(r, mg - 1)) * N + min(c, N - 8));
+          keep[i] = (u < nu && r < mg && c < N) ? 0xffffffffu : 0u;
         }
 #pragma unroll
-        for (int i = 0; i < RPL; ++i) {
-          float b[8];
-          bf16x8_to_f32(*(const uint4*)(Bg + (int64_t)min(krow[i], K - 1) * N + c0 + cc), b);
-#pragma unroll
-          for (int r = 0; r < NR; ++r) {
-            float s = acc[i][r];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) s = fmaf(b[e], d[r][e], s);
-            acc[i][r] = s;
+        for (int i = 0; i < 4; ++i) {
+          const int u = u0 + t + 256 * i;
+          if (u < nu) {
+            float v[8];
+            const uint4 x = raw[i];
+            bf16x8_to_f32(uint4{x.x & keep[i], x.y & keep[i], x.z & keep[i], x.w & keep[i]}, v);
+            float* d = Ds + (u / upr) * cw + 8 * (u % upr);
+            *(float4*)d = float4{v[0], v[1], v[2], v[3]};
+            *(float4*)(d + 4) = float4{v[4], v[5], v[6], v[7]};
           }
         }
       }
+      __syncthreads();
+      // this lane's column steps cc = cq + 256 s, s < ns; the next step's B
+      // loads are issued before this step's products (unconditional, clamped
+      // to the lane's last step), so two steps of loads are in flight
+      const int cn = min(cw, N - c0);
+      const int ns = cq < cn ? (cn - cq + 255) / 256 : 0;
+      if (ns > 0) {
+        const u16* bp[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) bp[i] = Bg + (int64_t)krow[i] * N + c0 + cq;
+        uint4 b0[RPL], b1[RPL];
+        auto load = [&](uint4 (&dst)[RPL], int s) {
+#pragma unroll
+          for (int i = 0; i < RPL; ++i) dst[i] = *(const uint4*)(bp[i] + 256 * min(s, ns - 1));
+        };
+        auto use = [&](const uint4 (&src)[RPL], int s) {
+          const int cc = cq + 256 * s;
+          float d[NR][8];
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const float4 d0 = *(const float4*)&Ds[r * cw + cc], d1 = *(const float4*)&Ds[r * cw + cc + 4];
+            d[r][0] = d0.x; d[r][1] = d0.y; d[r][2] = d0.z; d[r][3] = d0.w;
+            d[r][4] = d1.x; d[r][5] = d1.y; d[r][6] = d1.z; d[r][7] = d1.w;
+          }
+#pragma unroll
+          for (int i = 0; i < RPL; ++i) {
+            float b[8];
+            bf16x8_to_f32(src[i], b);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              float sum = acc[i][r];
+#pragma unroll
+              for (int e = 0; e < 8; ++e) sum = fmaf(b[e], d[r][e], sum);
+              acc[i][r] = sum;
+            }
+          }
+        };
+        load(b0, 0);
+        int s = 0;
+        for (; s + 1 < ns; s += 2) {
+          load(b1, s + 1);
+          use(b0, s);
+          load(b0, min(s + 2, ns - 1));
+          use(b1, s + 1);
+        }
+        if (s < ns) use(b0, s);
+      }
     }
+    const int kfirst = blockIdx.x * ROWS + 2 * RPL * w + (l >> 5);
 #pragma unroll
     for (int i = 0; i < RPL; ++i)
 #pragma unroll
@@ -245,8 +347,9 @@ __global__ void __launch_bounds__(256)
         v += __shfl_xor(v, 4);
         v += __shfl_xor(v, 8);
         v += __shfl_xor(v, 16);
-        if ((l & 31) == 0 && r < mg && krow[i] < K) {
-          const int64_t o = (off + r) * K + krow[i];
+        const int kr = kfirst + 2 * i;  // krow[i] before the clamp
+        if ((l & 31) == 0 && r < mg && kr < K) {
+          const int64_t o = (off + r) * K + kr;
           if (OUT_BF16) {
             const __bf16 h = (__bf16)v;
             ((u16*)dA)[o] = __builtin_bit_cast(unsigned short, h);
