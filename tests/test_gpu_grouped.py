@@ -219,3 +219,33 @@ def test_grouped_small_groups_every_bucket(bg, gpu):
         s, e = int(off[g]), int(off[g + 1])
         if e > s:
             assert rel(C32[s:e].cpu().numpy(), C[s:e]) <= 1e-5, (g, m_sizes[g])
+
+
+def test_grouped_small_groups_ragged_streams(bg, gpu):
+    """The streaming small-group kernels at ragged sizes: N = 1000 leaves the
+    forward's last 256-column block with 232 columns and dA's lanes with 3 or 4
+    column steps; K = 1096 gives the forward a second staged chunk of 72 k
+    (1024 + 72) and dA a last row block of 8 rows.  The 1- and 2-row buckets
+    hold enough groups (80 x 4 and 70 x 4 blocks) that the forward streams
+    them too.  C, dA, dB against FP64, per group."""
+    import torch
+    from batchgemm_amd.grouped import GroupedPlan
+    say("[grouped] ragged streams K=1096 N=1000")
+    m_sizes = [1] * 80 + [2] * 70 + [40]
+    rng = np.random.default_rng(11)
+    rng.shuffle(m_sizes)
+    m_sizes = [int(x) for x in m_sizes]
+    K, N = 1096, 1000
+    A, B, dC = make(m_sizes, K, N, seed=11)
+    p = GroupedPlan(m_sizes, K, N)
+    C32 = p.forward(A, B, out_dtype=torch.float32)
+    dA, dB = p.backward(A, B, dC, grad_dtype=torch.float32)
+    torch.cuda.synchronize()
+    C, rA, rB = ref64(A, B, dC, m_sizes)
+    off = np.concatenate([[0], np.cumsum(m_sizes)])
+    C32, dA, dB = C32.cpu().numpy(), dA.cpu().numpy(), dB.cpu().numpy()
+    for g in range(len(m_sizes)):
+        s, e = int(off[g]), int(off[g + 1])
+        assert rel(C32[s:e], C[s:e]) <= 1e-5, ("C", g, m_sizes[g])
+        assert rel(dA[s:e], rA[s:e]) <= 1e-5, ("dA", g, m_sizes[g])
+        assert rel(dB[g], rB[g]) <= 1e-5, ("dB", g, m_sizes[g])
