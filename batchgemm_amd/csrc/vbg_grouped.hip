@@ -99,9 +99,10 @@ __device__ __forceinline__ uint4 f32x8_to_bf16(const float (&f)[8]) {
 // k-phases are summed through LDS at the end.
 //   The B_g stream is pipelined: batch b+1 (UNR rows per lane) is loaded
 // before batch b is consumed, with the loads unconditional (clamped to the
-// chunk's last row) so every path has the same loads in flight; a block's
-// 1 MB (K = 2048, 256 columns) would otherwise take one round trip per batch
-// with nothing in flight between them.
+// chunk's last row) so every path has the same loads in flight.  On the
+// 1-row bucket of G = 128, K = 2048, N = 1408 this measured the same as one
+// batch at a time (129 vs 130 us, about 5.5 TB/s): that stream runs at the
+// HBM rate either way.
 template <int NR, bool OUT_BF16>
 __global__ void __launch_bounds__(256)
     vbg_grouped_fwd_small(const int* groups, int n_groups, const int* msize, const int64_t* rowoff,
