@@ -31,7 +31,45 @@ __global__ void __launch_bounds__(VBG_THREADS)
   const float* W = ws + ws_off[c.mat];
   const size_t mn = (size_t)d.M * d.N;
   if (d.ldc == d.M) {  // packed C: the panel index is the C index
-    for (int64_t e = c.begin + threadIdx.x; e < c.end; e += VBG_THREADS) {
+    // 4 consecutive elements per thread and step: one 16-byte load per slice
+    // (dword-aligned: panels start at any float offset), all slices' loads
+    // issued before the sum, which keeps the slice order of the scalar path.
+    // Grouped backward's dB (2048 x 1408, 6 slices): 37.7 -> 14.1 us.
+    const int64_t nv = (c.end - c.begin) / 4;
+    auto ld4 = [](const float* q) {  // 4-byte aligned 16-byte load
+      float4 v;
+      __builtin_memcpy(&v, q, sizeof v);
+      return v;
+    };
+    for (int64_t v = threadIdx.x; v < nv; v += VBG_THREADS) {
+      const int64_t e = c.begin + 4 * v;
+      float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      int k = 0;
+      for (; k + 4 <= d.ksplit; k += 4) {
+        float4 w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = ld4(W + (k + q) * mn + e);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          s.x += w[q].x;
+          s.y += w[q].y;
+          s.z += w[q].z;
+          s.w += w[q].w;
+        }
+      }
+      for (; k < d.ksplit; ++k) {
+        const float4 w = ld4(W + k * mn + e);
+        s.x += w.x;
+        s.y += w.y;
+        s.z += w.z;
+        s.w += w.w;
+      }
+      reduce_store(C, c.mat, (size_t)e, s.x, alpha, beta, out_bf16);
+      reduce_store(C, c.mat, (size_t)e + 1, s.y, alpha, beta, out_bf16);
+      reduce_store(C, c.mat, (size_t)e + 2, s.z, alpha, beta, out_bf16);
+      reduce_store(C, c.mat, (size_t)e + 3, s.w, alpha, beta, out_bf16);
+    }
+    for (int64_t e = c.begin + 4 * nv + threadIdx.x; e < c.end; e += VBG_THREADS) {
       float s = 0.0f;
       for (int k = 0; k < d.ksplit; ++k) s += W[k * mn + e];
       reduce_store(C, c.mat, (size_t)e, s, alpha, beta, out_bf16);
