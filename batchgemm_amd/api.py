@@ -12,6 +12,7 @@ Matrix convention for the tensor entry points: a column-major matrix with
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 from typing import Sequence
 
@@ -243,6 +244,12 @@ def sgemm_vbatched(As: Sequence[torch.Tensor], Bs: Sequence[torch.Tensor],
         lda.append(ld_of(a)); ldb.append(ld_of(b)); ldc.append(ld_of(c))
     plan = Plan(M, N, K, lda, ldb, ldc, transA, transB, prec, **opts)
     dev = Cs[0].device if len(Cs) else "cuda"
-    plan.run(pointer_array(As, dev), pointer_array(Bs, dev), pointer_array(Cs, dev), alpha, beta,
-             stream)
+    # The kernels read the pointer arrays asynchronously on `stream`: allocate
+    # them on that stream (the caching allocator then reuses their blocks only
+    # in that stream's order) and keep them alive on the plan.
+    ctx = torch.cuda.stream(stream) if isinstance(stream, torch.cuda.Stream) else contextlib.nullcontext()
+    with ctx:
+        ptrs = (pointer_array(As, dev), pointer_array(Bs, dev), pointer_array(Cs, dev))
+    plan._pointer_arrays = ptrs
+    plan.run(*ptrs, alpha, beta, stream)
     return plan

@@ -40,9 +40,15 @@ __device__ __forceinline__ int bswz(int row, int byte) {
 // byte offset is fixed once per work item and the k-tile position is a
 // wave-uniform scalar offset.  Rows past MN are clamped (to MN - 1 for KC,
 // to a 4-aligned row for MC); they only feed outputs that are never stored.
-// Reads past the operand return 0 (buffer range); k >= K, which can lie
-// inside the operand's allocation, is zeroed by selects when the tile that
-// straddles K is stored to LDS.
+// The scalar offset is not range-checked by the buffer unit, so a lane whose
+// k lies past the end of the item's k range has its vector offset pulled back
+// to the last k inside it (KC: to the last 8-aligned vector that holds a k
+// below the end; MC: to k = end - 1); those values are zeroed by selects when
+// the tile that straddles the end is stored to LDS.  The pull-back is a
+// no-op for every whole tile, so all tiles still issue the same loads.  A KC
+// vector that straddles K may run past the row's K: rows start 16-byte
+// aligned (ld % 8 == 0, 16-byte-aligned base), so that vector never leaves
+// the 16-byte block of its first element.
 template <int ROWS, int BK, int PITCH, bool KC>
 struct Bf16Loader {
   static constexpr int KB = ROWS * BK / (4 * VBG_THREADS);  // MC: k per lane
@@ -79,7 +85,13 @@ struct Bf16Loader {
   }
 
   __device__ __forceinline__ void init(const u16* p, int ld, int mn0, int MN, int K) {
-    rsrc = vbg_rsrc(p, (uint32_t)ld * (uint32_t)(KC ? MN : K) * 2u);
+    // the operand's span: KC rows x K (vectors of 8 k), MC K columns of MN
+    // (vectors of 4 mn) at ld
+    const uint32_t extent =
+        (MN <= 0 || K <= 0) ? 0u
+        : KC ? ((uint32_t)(MN - 1) * (uint32_t)ld + (uint32_t)((K + 7) & ~7)) * 2u
+             : ((uint32_t)(K - 1) * (uint32_t)ld + (uint32_t)((MN + 3) & ~3)) * 2u;
+    rsrc = vbg_rsrc(p, extent);
     ld2 = ld * 2;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -104,11 +116,21 @@ struct Bf16Loader {
     r.k0 = k0;
     r.kend = kend;
     if constexpr (KC) {
+      const int klast = (kend - 1) & ~7;  // last vector start with a k below kend
 #pragma unroll
-      for (int v = 0; v < NV; ++v) r.kc[v] = ld128(voff[v], k0 * 2);
+      for (int v = 0; v < NV; ++v) {
+        int row, kk;
+        coords(v, row, kk);
+        const int back = max(0, k0 + kk - klast);  // 0 unless the vector lies past kend
+        r.kc[v] = ld128(voff[v] - 2 * back, k0 * 2);
+      }
     } else {
+      int row, kk;
+      coords(0, row, kk);
+      // k-row of load j relative to k0: kk + j, clamped to the last one below kend
+      const int kmax = kend - 1 - k0;
 #pragma unroll
-      for (int j = 0; j < KB; ++j) r.mc[j] = ld64(voff[0], (k0 + j) * ld2);
+      for (int j = 0; j < KB; ++j) r.mc[j] = ld64(voff[0] + (min(kk + j, kmax) - kk) * ld2, k0 * ld2);
     }
   }
 

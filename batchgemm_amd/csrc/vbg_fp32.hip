@@ -235,7 +235,7 @@ struct Fp32SmallTile {
 // Reference-compatible VALU tile (32x32 outputs, 4 per thread).  Each output
 // is one fmaf chain over k = 0, 1, ..., K-1 in order.  With alpha = beta = 1
 // the chain is seeded from C, which is exactly the reference's correct tiles
-// (ROCm/framework/kernel_128_new.h:160-187, kernel_256_new.h:233-260);
+// (ROCm/framework/kernel_128_new.h:160-187, kernel_256_new.h:28-67, fmas :43-51);
 // otherwise it starts from 0 and C = alpha*acc + beta*C.
 template <bool TA, bool TB>
 struct SeqTile {

@@ -6,7 +6,7 @@
 //   fp16   : x * 2^e (per-matrix e: amax*2^e in [2^14, 2^15)) rounded to FP16,
 //            v_mfma_f32_32x32x16_f16, result scaled by 2^-(eA+eB)
 //   fp8    : x * (448/amax) rounded to OCP e4m3, v_mfma_f32_32x32x16_fp8_fp8,
-//            result scaled by 1/(sA*sB)
+//            result scaled by (1/sA)*(1/sB)
 // No counterpart in the reference (its kernels are FP32 only); the tiling,
 // work map and persistent driver are shared with the FP32 family.
 #include <type_traits>
@@ -632,28 +632,39 @@ __global__ void __launch_bounds__(VBG_THREADS)
   if (threadIdx.x == 0) atomicMax(&amax[2 * c.mat + c.op], __float_as_uint(m));
 }
 
-// scales[4b..4b+2] = {sA, sB, 1/(sA*sB)}
+// scales[4b..4b+2] = {sA, sB, post}, post the factor that undoes both scales.
+// FP16: s = 2^e, e = 15 - x for amax = f * 2^x (f in [0.5, 1)), capped at
+// 2^126 so a tiny amax keeps a finite scale; post = 2^-(eA + eB) exactly
+// (the product sA*sB itself would overflow once eA + eB >= 128, i.e. for
+// amaxA * amaxB below about 2^-98).  FP8: s = 448/amax, capped at 2^126;
+// post = (1/sA) * (1/sB), each factor in range.
 __global__ void __launch_bounds__(VBG_THREADS)
     vbg_scales_kernel(int prec, int batch, const unsigned* amax, float* scales) {
   const int b = blockIdx.x * VBG_THREADS + threadIdx.x;
   if (b >= batch) return;
-  float s[2];
+  float s[2], inv[2];
+  int e[2];
 #pragma unroll
   for (int op = 0; op < 2; ++op) {
     const float m = __uint_as_float(amax[2 * b + op]);
+    e[op] = 0;
+    s[op] = 1.0f;
+    inv[op] = 1.0f;
     if (!(m > 0.0f) || isinf(m)) {
-      s[op] = 1.0f;
+      // no finite nonzero entry: unscaled
     } else if (prec == VBG_PREC_FP16_SCALED) {
       int x;
       frexpf(m, &x);  // m = f * 2^x, f in [0.5, 1)
-      s[op] = ldexpf(1.0f, 15 - x);
+      e[op] = min(15 - x, 126);
+      s[op] = ldexpf(1.0f, e[op]);
     } else {
-      s[op] = 448.0f / m;
+      s[op] = fminf(448.0f / m, 0x1p126f);
+      inv[op] = 1.0f / s[op];
     }
   }
   scales[4 * b] = s[0];
   scales[4 * b + 1] = s[1];
-  scales[4 * b + 2] = 1.0f / (s[0] * s[1]);
+  scales[4 * b + 2] = prec == VBG_PREC_FP16_SCALED ? ldexpf(1.0f, -(e[0] + e[1])) : inv[0] * inv[1];
   scales[4 * b + 3] = 0.0f;
 }
 

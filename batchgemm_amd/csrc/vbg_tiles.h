@@ -80,11 +80,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t vbg_rsrc(const void* p, uint32
 //
 // Addressing: init() fixes each lane's byte offset for its row and k-quad once
 // per work item (rows past MN clamped to MN-1: they only feed outputs that are
-// never stored); a k-tile's position is a wave-uniform scalar offset, so the
-// main loop issues no vector address arithmetic.  In the last (partial) k-tile
-// every k >= K reads as zero (out-of-range loads return 0 from the buffer
-// unit; the select guards data that lies inside the allocation).  Operands
-// must span less than 2^31 bytes (the plan checks).
+// never stored); a whole k-tile's position is a wave-uniform scalar offset, so
+// the main loop issues no vector address arithmetic.  The scalar offset is not
+// range-checked by the buffer unit, so it is only used where every address is
+// inside the operand.  The last (partial) k-tile addresses each element with
+// its k clamped to K-1, entirely in the range-checked vector offset, and
+// zeroes k >= K by a select: no load leaves the operand, whatever lies after
+// it.  The buffer extent is the operand's own span (last element + 1).
+// Operands must span less than 2^31 bytes (the plan checks).
 template <int ROWS, int BK, int P, bool KC>
 struct TileLoader {
   static constexpr int NV = ROWS * BK / (4 * VBG_THREADS);  // float4 groups per thread
@@ -115,7 +118,11 @@ struct TileLoader {
 
   // p: operand base; MN rows; K the operand's full reduction length.
   __device__ __forceinline__ void init(const float* p, int ld, int mn0, int MN, int K) {
-    const uint32_t extent = (uint32_t)ld * (uint32_t)(KC ? MN : K) * 4u;
+    // k contiguous: rows x K at ld; mn contiguous: K columns of MN at ld
+    const uint32_t extent =
+        (MN <= 0 || K <= 0) ? 0u
+        : KC ? ((uint32_t)(MN - 1) * (uint32_t)ld + (uint32_t)K) * 4u
+             : ((uint32_t)(K - 1) * (uint32_t)ld + (uint32_t)MN) * 4u;
     rsrc = vbg_rsrc(p, extent);
     ld4 = ld * 4;
 #pragma unroll
@@ -141,7 +148,8 @@ struct TileLoader {
     }
   }
 
-  // The last k-tile [k0, K): k >= K reads as zero.
+  // The last k-tile [k0, K) (K: the end of the item's k range, at most the
+  // operand's K): k >= K reads as zero, and its address is that of k = K-1.
   __device__ __forceinline__ void load_tail(Regs& r, int k0, int K) const {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -149,11 +157,11 @@ struct TileLoader {
       coords(v, row, kq);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const bool ok = k0 + 4 * kq + e < K;
-        const unsigned bits = KC ? __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff[v] + 4 * e, k0 * 4, 0)
-                                 : __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff[v], (k0 + e) * ld4, 0);
+        const int k = k0 + 4 * kq + e;
+        const int dk = min(k, K - 1) - 4 * kq;  // k offset from the lane's k-quad origin
+        const unsigned bits = __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff[v] + dk * (KC ? 4 : ld4), 0, 0);
         const float x = __builtin_bit_cast(float, bits);
-        r[v][e] = ok ? x : 0.0f;
+        r[v][e] = k < K ? x : 0.0f;
       }
     }
   }

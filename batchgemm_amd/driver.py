@@ -70,11 +70,8 @@ def main(argv=None) -> int:
         vb.b(i).fill_(i + 2)
         vb.c(i).fill_(i + 3)
     if args.per_matrix_library:
-        mats = [(vb.a(i), vb.b(i), vb.c(i)) for i in range(B)]
-
-        def run():  # C_i <- C_i + A_i B_i, one library call per matrix
-            for a, b, c in mats:
-                torch.addmm(c, a, b, beta=1.0, alpha=1.0, out=c)
+        from .baseline import library_loop
+        run = library_loop(vb, 1.0, 1.0)  # C_i <- C_i + A_i B_i, one library call per matrix
         plan = None
     else:
         plan = Plan(M, N, K, prec=args.prec, policy=args.policy,

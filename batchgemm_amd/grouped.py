@@ -84,7 +84,12 @@ _plans: dict = {}
 
 
 def grouped_plan(m_sizes, K: int, N: int) -> GroupedPlan:
-    key = (tuple(int(m) for m in m_sizes), int(K), int(N), torch.cuda.current_device())
+    """The cached plan for these group sizes on the current device and stream.
+    A plan's device pointer arrays and work-queue counters are per-run state,
+    so calls that may overlap in time (different streams) get different
+    plans; calls on one stream run in order and share one."""
+    key = (tuple(int(m) for m in m_sizes), int(K), int(N), torch.cuda.current_device(),
+           int(torch.cuda.current_stream().cuda_stream))
     p = _plans.get(key)
     if p is None:
         p = _plans[key] = GroupedPlan(m_sizes, K, N)

@@ -185,6 +185,7 @@ class VBatch:
 
 
 def dataset_path() -> str:
-    """The reference dataset (4096 "M N K" lines) shipped as a test fixture."""
-    here = os.path.dirname(os.path.abspath(__file__))
-    return os.path.join(os.path.dirname(here), "tests", "golden", "data_MN_K_512_128")
+    """The reference's dataset (4096 "M N K" lines, ROCm/data/data_MN_K_512_128),
+    shipped as package data: the benchmark driver's default input, as the
+    reference's driver reads ../data/data_MN_K_512_128 (ROCm/framework/gemm.cuh:35)."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "data_MN_K_512_128")

@@ -11,7 +11,7 @@
  *                                    wavefront accounting, demotion cascade,
  *                                    bounding-box grid
  *   ROCm/framework/kernel_128_new.h:131-201  gemm_128_32x32 (column-major tile)
- *   ROCm/framework/kernel_256_new.h:207-277  gemm_256_32x64 (row-major tile):
+ *   ROCm/framework/kernel_256_new.h:2-72  gemm_256_32x64 (row-major tile):  
  *                                    per element a sequential fma chain over
  *                                    ascending k, seeded from C
  *   ROCm/default/gemm.cuh:82         hipblasSgemm(OP_N, OP_N, ..., lda=M,
@@ -303,9 +303,9 @@ int orc_sgemm_vbatched(char ta, char tb, int batch, const int* M, const int* N, 
 /* Reference tile semantics (a12 gemm_128_32x32, a13 gemm_256_32x64): each    */
 /* output element is C <- fma(a_k, b_k, C) for k = 0, 1, ..., K-1, seeded     */
 /* with the old C, no alpha/beta (kernel_128_new.h:160-187 and                */
-/* kernel_256_new.h:233-260).  rowmajor = 0: lda=M, ldb=K, ldc=M;             */
-/* rowmajor = 1: lda=K, ldb=N, ldc=N.  Uses fmaf, so this must be built with  */
-/* -ffp-contract=off to keep every other operation unfused.                   */
+/* kernel_256_new.h:28-67, fmas :43-51).  rowmajor = 0: lda=M, ldb=K, ldc=M; */
+/* rowmajor = 1: lda=K, ldb=N, ldc=N.  Its only arithmetic is the explicit    */
+/* fmaf chain, so the result does not depend on the compiler's contraction.  */
 int orc_ref_tile_seq(int rowmajor, int M, int N, int K, const float* A, const float* B, float* C) {
   if (M < 0 || N < 0 || K < 0) return ORC_ERR_ARG;
   for (int i = 0; i < M; ++i)

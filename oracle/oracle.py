@@ -8,7 +8,7 @@ Two layers:
   * ``liboracle.so`` (``oracle.c``) -- the C restatement of the reference path:
     dataset ingest and FLOP accounting (ROCm/framework/gemm.cuh:34-48,112-116),
     the planner (gemm.cuh:121-229), the sequential-fma tile semantics
-    (ROCm/framework/kernel_128_new.h:131-201, kernel_256_new.h:207-277), an FP64
+    (ROCm/framework/kernel_128_new.h:131-201, kernel_256_new.h:2-72), an FP64
     column-major GEMM with hipBLAS conventions (ROCm/default/gemm.cuh:82) and the
     host per-matrix SGEMM loop used as the CPU baseline.
   * numpy emulation of the operand quantisation used by the split-precision
@@ -27,7 +27,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ORC_LIBRARY selects another build of oracle.c (the sanitized host test run
+# loads liboracle_asan.so this way)
+_LIB_PATH = os.environ.get("ORC_LIBRARY") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 ORC_OK = 0
@@ -197,14 +199,18 @@ def bf16_split(x):
     return hi, lo
 
 
+FP16_EXP_MAX = 126  # scale 2**e stays a finite float32 for a tiny amax
+
+
 def fp16_pow2_exponent(amax: float) -> int:
     """Per-matrix scale exponent e with amax * 2**e in [2**14, 2**15):
-    amax = f * 2**x with f in [0.5, 1)  ->  e = 15 - x.  amax == 0 -> 0."""
+    amax = f * 2**x with f in [0.5, 1)  ->  e = 15 - x, capped at 126 (amax
+    below 2**-111 keeps a finite scale).  amax == 0 -> 0."""
     amax = float(np.float32(amax))
     if amax == 0.0 or not np.isfinite(amax):
         return 0
     _, x = np.frexp(np.float32(amax))
-    return int(15 - x)
+    return int(min(15 - x, FP16_EXP_MAX))
 
 
 def fp16_scaled(x, e: int):
@@ -216,12 +222,33 @@ def fp16_scaled(x, e: int):
 E4M3_MAX = 448.0
 
 
+FP8_SCALE_MAX = np.float32(2.0 ** 126)
+
+
 def fp8_scale(amax: float) -> np.float32:
-    """Per-matrix FP8 scale s = 448 / amax in float32 (amax == 0 -> 1)."""
+    """Per-matrix FP8 scale s = 448 / amax in float32, capped at 2**126
+    (amax == 0 -> 1)."""
     amax = np.float32(amax)
     if amax == 0 or not np.isfinite(amax):
         return np.float32(1.0)
-    return np.float32(np.float32(E4M3_MAX) / amax)
+    with np.errstate(over="ignore"):
+        s = np.float32(np.float32(E4M3_MAX) / amax)
+    return np.float32(min(s, FP8_SCALE_MAX))
+
+
+def post_scale(prec: str, a, b) -> float:
+    """The factor that undoes both operand scales, as the scaled paths apply
+    it to the FP32 result.  FP16: a, b are the exponents eA, eB and the factor
+    is 2**-(eA + eB), computed exactly in FP64 and rounded once to float32
+    (a power of two: exact down to 2**-149).  FP8: a, b are the scales sA, sB
+    and the factor is (1/sA) * (1/sB) with each step rounded to float32."""
+    if prec == "fp16":
+        return float(np.float32(np.ldexp(1.0, -(int(a) + int(b)))))
+    if prec == "fp8":
+        ia = np.float32(np.float32(1.0) / np.float32(a))
+        ib = np.float32(np.float32(1.0) / np.float32(b))
+        return float(np.float32(ia * ib))
+    raise ValueError(prec)
 
 
 def e4m3_round(x):
@@ -243,6 +270,28 @@ def e4m3_round(x):
 def fp8_scaled(x, s):
     xs = (np.asarray(x, np.float32) * np.float32(s)).astype(np.float32)
     return e4m3_round(xs)
+
+
+def split_product(prec: str, A, B):
+    """op(A) @ op(B) as a split-precision path computes it, up to its FP32
+    accumulation: the operands quantised as the kernels quantise them (bf16x3:
+    hi/lo split, C = hi*hi + hi*lo + lo*hi; fp16 / fp8: per-matrix scale from
+    each operand's amax, then rounding), multiplied exactly in FP64 and scaled
+    back by the kernels' post-scale.  A, B: float32 arrays (M x K, K x N)."""
+    A = np.asarray(A, np.float32); B = np.asarray(B, np.float32)
+    d = lambda x: x.astype(np.float64)
+    if prec == "bf16x3":
+        ah, al = bf16_split(A); bh, bl = bf16_split(B)
+        return d(ah) @ d(bh) + d(ah) @ d(bl) + d(al) @ d(bh)
+    amax_a = float(np.abs(A).max()) if A.size else 0.0
+    amax_b = float(np.abs(B).max()) if B.size else 0.0
+    if prec == "fp16":
+        ea, eb = fp16_pow2_exponent(amax_a), fp16_pow2_exponent(amax_b)
+        return (d(fp16_scaled(A, ea)) @ d(fp16_scaled(B, eb))) * post_scale("fp16", ea, eb)
+    if prec == "fp8":
+        sa, sb = fp8_scale(amax_a), fp8_scale(amax_b)
+        return (d(fp8_scaled(A, sa)) @ d(fp8_scaled(B, sb))) * post_scale("fp8", sa, sb)
+    raise ValueError(prec)
 
 
 def rel_fro(x, ref) -> float:

@@ -5,7 +5,8 @@
   are compared against them);
 - data_MN_K_512_128 and data_MN_K_512_128_sort: the reference's dataset and
   its sorted first 256 lines, as shipped (ROCm/data/; the sorted file is the
-  output of ROCm/data/sort.cpp, which pins the Wang ordering).
+  output of ROCm/data/sort.cpp, which pins the Wang ordering).  The dataset
+  is also copied to batchgemm_amd/data/, the benchmark driver's default input.
 
 Run in the build container (the reference tree is not present elsewhere):
     python tests/golden/make_golden.py
@@ -59,6 +60,11 @@ def main():
         f.write("\n")
     for name in ("data_MN_K_512_128", "data_MN_K_512_128_sort"):
         shutil.copyfile(os.path.join(REF, "ROCm", "data", name), os.path.join(HERE, name))
+    # the benchmark driver's default dataset, shipped with the package
+    pkg_data = os.path.join(os.path.dirname(os.path.dirname(HERE)), "batchgemm_amd", "data")
+    os.makedirs(pkg_data, exist_ok=True)
+    shutil.copyfile(os.path.join(REF, "ROCm", "data", "data_MN_K_512_128"),
+                    os.path.join(pkg_data, "data_MN_K_512_128"))
 
 
 if __name__ == "__main__":

@@ -40,21 +40,7 @@ def op_matrix(vb, which, b):
 
 def emulate(orc, prec, A, B):
     """Quantised operands and the post-scale, as the kernel builds them."""
-    if prec == "bf16x3":
-        ah, al = orc.bf16_split(A); bh, bl = orc.bf16_split(B)
-        d = lambda x: x.astype(np.float64)
-        return d(ah) @ d(bh) + d(ah) @ d(bl) + d(al) @ d(bh)
-    if prec == "fp16":
-        ea = orc.fp16_pow2_exponent(np.abs(A).max() if A.size else 0.0)
-        eb = orc.fp16_pow2_exponent(np.abs(B).max() if B.size else 0.0)
-        qa = orc.fp16_scaled(A, ea).astype(np.float64); qb = orc.fp16_scaled(B, eb).astype(np.float64)
-        return (qa @ qb) * np.float64(np.float32(1.0) / (np.float32(2.0 ** ea) * np.float32(2.0 ** eb)))
-    if prec == "fp8":
-        sa = orc.fp8_scale(np.abs(A).max() if A.size else 0.0)
-        sb = orc.fp8_scale(np.abs(B).max() if B.size else 0.0)
-        qa = orc.fp8_scaled(A, sa).astype(np.float64); qb = orc.fp8_scaled(B, sb).astype(np.float64)
-        return (qa @ qb) * np.float64(np.float32(1.0) / (sa * sb))
-    raise ValueError(prec)
+    return orc.split_product(prec, A, B)
 
 
 FP64_BOUND = {"bf16x3": 1e-5, "fp16": 2e-3, "fp8": 0.2}
@@ -62,7 +48,7 @@ EMUL_BOUND = {"bf16x3": 2e-6, "fp16": 2e-6, "fp8": 2e-4}
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16", "fp8"])
-@pytest.mark.parametrize("trans", ["NN", "TN", "NT"])
+@pytest.mark.parametrize("trans", ["NN", "TN", "NT", "TT"])
 @pytest.mark.parametrize("outliers", [0.0, 0.01])
 def test_split_precision_paths(bg, orc, gpu, prec, trans, outliers):
     import torch
@@ -212,3 +198,28 @@ def test_scaled_paths_padded_leading_dimension(bg, orc, gpu, prec, trans):
         got = vb.host_flat("C", b).reshape(N[b], M[b]).T.astype(np.float64)
         emul = emulate(orc, prec, A, B) + 0.5 * c0
         assert orc.rel_fro(got, emul) <= EMUL_BOUND[prec], b
+
+
+@pytest.mark.parametrize("prec", ["fp16", "fp8"])
+def test_scaled_paths_tiny_magnitudes(bg, orc, gpu, prec):
+    """Operands near 2^-58: the product of the two operand scales (2^67 each
+    for FP16, about 2^65 for FP8) overflows FP32, so a post-scale computed as
+    1/(sA*sB) would be 1/inf = 0 and C would silently come out zero.  The
+    result itself (about 2^-110) is an ordinary FP32 number, and the kernels
+    must produce it: FP16 applies 2^-(eA+eB) exactly, FP8 (1/sA)*(1/sB)."""
+    import torch
+    from batchgemm_amd.workloads import VBatch
+    say(f"[{prec}] tiny magnitudes")
+    M, N, K = [64, 40, 130], [64, 96, 33], [128, 70, 257]
+    vb = VBatch(M, N, K).fill("normal", seed=17)
+    vb.A.mul_(2.0 ** -58)
+    vb.B.mul_(2.0 ** -58)
+    bg.Plan(M, N, K, prec=prec).run(vb.dA, vb.dB, vb.dC, 1.0, 0.0)
+    torch.cuda.synchronize()
+    for b in range(vb.batch):
+        A = op_matrix(vb, "A", b); B = op_matrix(vb, "B", b)
+        got = vb.host_flat("C", b).reshape(N[b], M[b]).T.astype(np.float64)
+        exact = A.astype(np.float64) @ B.astype(np.float64)
+        assert np.abs(got).max() > 0.0, "post-scale flushed the result to zero"
+        assert orc.rel_fro(got, emulate(orc, prec, A, B)) <= EMUL_BOUND[prec], b
+        assert orc.rel_fro(got, exact) <= FP64_BOUND[prec], b

@@ -322,3 +322,12 @@ def test_li_policy_matches_oracle(bg, orc):
     for m, n, k, th in cases:
         p = bg.Plan(m, n, k, host_only=True, policy="li", tlp_thres=th)
         assert np.array_equal(p.tiles(), orc.li_plan(m, n, th)), (len(m), th)
+
+
+def test_package_dataset_is_the_reference_dataset(bg):
+    """The driver's default input (package data) is byte for byte the dataset
+    fixture copied from the reference (ROCm/data/data_MN_K_512_128)."""
+    from batchgemm_amd import workloads
+    path = workloads.dataset_path()
+    assert os.path.dirname(path).endswith(os.path.join("batchgemm_amd", "data"))
+    assert open(path, "rb").read() == open(DATA, "rb").read()

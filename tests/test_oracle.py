@@ -242,6 +242,52 @@ def test_fp16_scaled_matches_numpy():
     assert np.array_equal(q, (x * np.float32(2.0 ** e)).astype(np.float16).astype(np.float32))
 
 
+def test_post_scale_undoes_both_scales():
+    """FP16: 2^-(eA+eB) is exact wherever float32 holds it, including where
+    the product of the two scales 2^eA * 2^eB overflows float32 (eA + eB >=
+    128: tiny operands).  FP8: (1/sA)(1/sB) is 1/(sA sB) to float32 rounding,
+    also where sA * sB overflows."""
+    from oracle import oracle as o
+    for ea, eb in [(0, 0), (14, -3), (70, 70), (126, 20), (-60, -60)]:
+        p = o.post_scale("fp16", ea, eb)
+        assert p > 0.0 and p * 2.0 ** ea * 2.0 ** eb == 1.0
+    assert o.post_scale("fp16", 126, 126) == 0.0  # 2^-252: below float32's range
+    for sa, sb in [(448.0, 1.0), (3.7, 0.01), (2.0 ** 65, 2.0 ** 64.5), (1e-20, 7.0)]:
+        sa, sb = float(np.float32(sa)), float(np.float32(sb))
+        p = o.post_scale("fp8", sa, sb)
+        # two float32 roundings: within 2 units in the last place of p (a
+        # subnormal p, as for the 2^65 scales, has a coarser last place)
+        assert p > 0.0 and abs(p - 1.0 / (sa * sb)) <= 2 * float(np.spacing(np.float32(p)))
+    # scales stay finite for tiny operands
+    assert o.fp16_pow2_exponent(2.0 ** -140) == 126
+    assert np.isfinite(o.fp8_scale(1e-40)) and o.fp8_scale(1e-40) == np.float32(2.0 ** 126)
+
+
+def test_split_product_exact_on_representable_inputs():
+    """Where every quantisation step is exact the emulated product is the
+    exact product: bf16x3 on values with 16 significant bits (hi + lo holds
+    them, lo*lo = 0 is the only dropped term and vanishes when lo has no
+    bits below the product's); FP16 on small integers (amax 7 -> scale 2^12,
+    7 * 2^12 < 2^15); FP8 on integers in [-7, 7] with amax 7 (scale 448/7 = 64,
+    and 64 k, |k| <= 7, is representable in e4m3)."""
+    from oracle import oracle as o
+    rng = np.random.default_rng(8)
+    A = rng.integers(-7, 8, (13, 40)).astype(np.float32); A[0, 0] = 7
+    B = rng.integers(-7, 8, (40, 9)).astype(np.float32); B[0, 0] = -7
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    for prec in ("fp16", "fp8", "bf16x3"):
+        assert np.array_equal(o.split_product(prec, A, B), exact), prec
+    # bf16x3 keeps 16 significant bits of each operand: hi + lo = x exactly
+    x = (rng.integers(1 << 15, 1 << 16, (6, 30)) * 2.0 ** -20).astype(np.float32)
+    y = (rng.integers(1 << 15, 1 << 16, (30, 5)) * 2.0 ** -3).astype(np.float32)
+    hi, lo = o.bf16_split(x)
+    assert np.array_equal(hi.astype(np.float64) + lo, x.astype(np.float64))
+    full = x.astype(np.float64) @ y.astype(np.float64)
+    xh, xl = o.bf16_split(x); yh, yl = o.bf16_split(y)
+    dropped = xl.astype(np.float64) @ yl.astype(np.float64)
+    assert np.allclose(o.split_product("bf16x3", x, y) + dropped, full, rtol=1e-15, atol=0)
+
+
 def test_wang_order_matches_reference_sorted_dataset(orc):
     """sort.cpp's output for the first 256 dataset lines is shipped with the
     reference (ROCm/data/data_MN_K_512_128_sort)."""
