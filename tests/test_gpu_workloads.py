@@ -1,36 +1,51 @@
 """GPU parity at the benchmark workloads' own sizes (needs an MI355X).
 
 The unbalanced batch: B = 2048 column-major GEMMs, 90% with M,N,K ~ U{16..128}
-and 10% with M,N,K ~ U{512..2048} (the plan the benchmark times: about 24,750
-work items, split-K on the largest matrices, several launches), alpha = 1.5,
-beta = 0.5, C ~ U[-1,1], A,B ~ N(0,1) with 1% of the entries scaled by 1e3
-(the precision workload's inputs).  Every path runs on that batch -- FP32,
-bf16x3, FP16 with per-matrix 2^e scaling and FP8 e4m3 with per-matrix amax
-scaling, the last two both on packed operands and quantised in the k-loop --
-and is checked on every large matrix (all of M,N,K in 512..2048) plus 64
-small ones, each against FP64 products of the same inputs (numpy float64 on
-the host, the product that pins the oracle's orc_dgemm).
+and 10% with M,N,K ~ U{512..2048} (the plan the benchmark times: 24,750 work
+items in a 128x128 launch and a mixed-shape launch), alpha = 1.5, beta = 0.5,
+C ~ U[-1,1], A,B ~ N(0,1) with 1% of the entries scaled by 1e3 (the precision
+workload's inputs).  Every path runs on that batch -- FP32, bf16x3, FP16 with
+per-matrix 2^e scaling and FP8 e4m3 with per-matrix amax scaling, the last
+two both on packed operands and quantised in the k-loop -- and is checked on
+every large matrix (all of M,N,K in 512..2048) plus 64 small ones, each
+against FP64 products of the same inputs (numpy float64 on the host, the
+product that pins the oracle's orc_dgemm).  (This batch needs no split-K:
+every launch has far more items than workgroup slots; split-K is covered by
+the forced-split tests of every tile.)
 
-Bounds.  FP32 and bf16x3: relative Frobenius error <= 1e-5 against FP64 (the
-path's target).  FP16 and FP8 against the oracle's emulation of their
-operand quantisation (the same quantised operands multiplied exactly in FP64,
-times the kernels' post-scale): the only difference left is the MFMA's FP32
-accumulation, bounded per matrix by the probabilistic summation bound
-(Higham & Mary 2019)
-
-    err <= c * u * sqrt(K) * rho,  u = 2^-24,  rho = ||(|A| |B|)||_F / ||A B||_F
-
-(rho computed from the FP64 products of the unquantised operands), with c = 4
-for FP16 and c = 64 for FP8: the gfx950 FP8 MFMA's own accumulation differs
-from exact-product FP32 summation by about 16x the FP32 figure (measured at K
-<= 257, where the identity test in test_gpu_precision.py shows the operand
-quantisation itself exact element by element).  At K = 2048 with N(0,1)
-operands rho is about 29, so the bound is about 3.1e-4 (FP16) and 5e-3 (FP8);
-the measured errors are printed next to it.  Against FP64 the formats' own
-rounding is bounded loosely (FP16 2e-3, FP8 0.2) and printed.
+Bounds, per matrix, with ref = alpha*A*B + beta*C0, R = ||alpha |A| |B|||_F /
+||ref||_F and u = 2^-24:
+  * FP32: 1e-5 (the target; FP32 accumulation gives about u*sqrt(K)*R).
+  * bf16x3: the format's own error -- lo = bf16(x - hi) keeps 16 significant
+    bits of x, so each product carries at most 2*2^-18 from the two lo
+    roundings plus the dropped lo*lo term (<= 2^-18) -- plus FP32
+    accumulation: (3*2^-18 + 4*u*sqrt(K)) * R.  A small matrix dominated by a
+    few 1e3-scaled products can exceed 1e-5 on the format alone (about 1% of
+    21 x 38 x 115 outlier matrices do, emulated exactly); the path's target,
+    <= 1e-5, is asserted on the pooled error over all checked matrices.
+  * FP16 against the oracle's emulation of its quantisation (same quantised
+    operands multiplied exactly in FP64, times the kernels' post-scale): the
+    FP16 MFMA forms exact products and accumulates in FP32 (crafted sums on
+    the GPU round exactly as FP32 does), so the difference is FP32 summation,
+    bounded by the probabilistic bound (Higham & Mary 2019) 4*u*sqrt(K)*R_q,
+    R_q as R on the quantised operands (R is used: the two differ by the
+    quantisation only).
+  * FP8 against the emulation: the gfx950 FP8 MFMA sums each lane's 8
+    products at reduced precision before its FP32 accumulation -- crafted sums
+    lose a product of 2^-16 times the group's largest (and seven of them, 2^-13
+    together) while the same products in another group, or across MFMAs,
+    add exactly as FP32 -- so each group of 8 can be off by up to about 2^-13
+    of its largest product, with signs that vary.  Per element that is at most
+    about 2^-13 * sqrt(sum_k (a_ik b_kj)^2) in the root-sum-square sense, and
+    the bound is 4 * 2^-13 * sqrt(sum((A*A) @ (B*B))) / ||ref||_F -- K-free, as
+    the measured error is (about 1.5e-5 relative on N(0,1) inputs at K = 16 ..
+    2048, against a bound of about 5e-4 there).
+  * FP16 / FP8 against FP64: the formats' own rounding, bounded loosely (2e-3,
+    0.2) and printed.
 
 The batching workload: one batch of 50 (B ~ U{256..8192}, M,N,K log-uniform
-in [8, 1024], U[-1,1] entries) in bf16x3, every matrix against FP64, <= 1e-5.
+in [8, 1024], U[-1,1] entries) in bf16x3, every matrix against FP64 with the
+bf16x3 bound above, and <= 1e-5 pooled.
 """
 import sys
 import time
@@ -47,6 +62,10 @@ U = 2.0 ** -24
 def say(*a):
     print(*a, flush=True)
     sys.stdout.flush()
+
+
+def bf16x3_bound(K, R):
+    return (3 * 2.0 ** -18 + 4 * U * np.sqrt(float(K))) * R
 
 
 class Unbalanced:
@@ -67,15 +86,18 @@ class Unbalanced:
         self.nbig = len(big)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        self.A, self.B, self.c0, self.exact, self.rho = {}, {}, {}, {}, {}
+        self.A, self.B, self.ref, self.R, self.sq = {}, {}, {}, {}, {}
+        self.c0 = {}
         for b in self.sel:
             A = self.vb.a(b).cpu().numpy().astype(np.float64)
             B = self.vb.b(b).cpu().numpy().astype(np.float64)
             self.A[b], self.B[b] = A.astype(np.float32), B.astype(np.float32)
             self.c0[b] = self.vb.c(b).cpu().numpy().astype(np.float64)
-            P = A @ B
-            self.exact[b] = P
-            self.rho[b] = np.linalg.norm(np.abs(A) @ np.abs(B)) / max(np.linalg.norm(P), 1e-300)
+            ref = ALPHA * (A @ B) + BETA * self.c0[b]
+            nref = np.linalg.norm(ref)
+            self.ref[b] = ref
+            self.R[b] = ALPHA * np.linalg.norm(np.abs(A) @ np.abs(B)) / nref
+            self.sq[b] = ALPHA * np.sqrt(np.sum((A * A) @ (B * B))) / nref
         say(f"[workload] unbalanced B=2048: {self.nbig} large + {len(small)} small matrices checked; "
             f"FP64 references {time.perf_counter() - t0:.1f} s")
 
@@ -85,8 +107,7 @@ class Unbalanced:
         self.vb.C.copy_(self.C0)
         plan = bg.Plan(self.M, self.N, self.K, prec=prec, **opts)
         kinds = [bg.Plan.KERNEL_KINDS[int(k)] for k in plan.kernels()[:, 0]]
-        say(f"[workload] {prec} {opts}: {plan.n_items} items, kernels {kinds}, "
-            f"{int((plan.ksplit() > 1).sum())} split-K matrices")
+        say(f"[workload] {prec} {opts}: {plan.n_items} items, kernels {kinds}")
         plan.run(self.vb.dA, self.vb.dB, self.vb.dC, ALPHA, BETA)
         torch.cuda.synchronize()
         return plan
@@ -102,26 +123,27 @@ def unbalanced(bg, gpu):
     del w.vb
 
 
-def check_vs_fp64(w, prec, bound):
-    worst = (0.0, None)
-    for b in w.sel:
-        ref = ALPHA * w.exact[b] + BETA * w.c0[b]
-        e = float(np.linalg.norm(w.got(b) - ref) / np.linalg.norm(ref))
-        if e > worst[0]:
-            worst = (e, b)
-        assert e <= bound, (prec, int(b), int(w.M[b]), int(w.N[b]), int(w.K[b]), e)
-    b = worst[1]
-    say(f"  {prec}: max rel err vs FP64 {worst[0]:.3g} (matrix {b}: "
-        f"{w.M[b]}x{w.N[b]}x{w.K[b]}), bound {bound:g}")
-
-
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_unbalanced_batch_fp32_and_bf16x3_vs_fp64(unbalanced, prec):
-    plan = unbalanced.run(prec)
-    if prec == "fp32":  # the benchmark's plan: split-K reduction and several launches
-        assert (plan.ksplit() > 1).any() and plan.n_launches >= 2
-        assert plan.n_items > 20000
-    check_vs_fp64(unbalanced, prec, 1e-5)
+    w = unbalanced
+    plan = w.run(prec)
+    if prec == "fp32":  # the benchmark's plan: the 128x128 launch and the mixed one
+        assert plan.n_launches >= 2 and plan.n_items > 20000
+    num = den = 0.0
+    worst = (0.0, None, 0.0)
+    for b in w.sel:
+        d = np.linalg.norm(w.got(b) - w.ref[b]); n = np.linalg.norm(w.ref[b])
+        num += d * d; den += n * n
+        e = d / n
+        bound = 1e-5 if prec == "fp32" else bf16x3_bound(w.K[b], w.R[b])
+        if e / bound > worst[0]:
+            worst = (e / bound, b, e)
+        assert e <= bound, (prec, int(b), int(w.M[b]), int(w.N[b]), int(w.K[b]), e, bound)
+    pooled = (num / den) ** 0.5
+    r, b, e = worst
+    say(f"  {prec}: pooled rel err vs FP64 {pooled:.3g}; worst matrix {b} ({w.M[b]}x{w.N[b]}x{w.K[b]}) "
+        f"err {e:.3g} = {r:.3g} of its bound")
+    assert pooled <= 1e-5
 
 
 @pytest.mark.parametrize("prec", ["fp16", "fp8"])
@@ -130,24 +152,25 @@ def test_unbalanced_batch_scaled_paths_vs_emulation(orc, unbalanced, prec, pack)
     w = unbalanced
     plan = w.run(prec, pack_operands=pack)
     assert (plan.packed_bytes > 0) == (pack > 0)
-    c = {"fp16": 4.0, "fp8": 64.0}[prec]
-    worst_ratio = (0.0, None, 0.0, 0.0)
+    worst = (0.0, None, 0.0, 0.0)
     worst_fp64 = 0.0
     for b in w.sel:
         got = w.got(b)
         emul = ALPHA * orc.split_product(prec, w.A[b], w.B[b]) + BETA * w.c0[b]
         e = float(np.linalg.norm(got - emul) / np.linalg.norm(emul))
-        bound = c * U * np.sqrt(float(w.K[b])) * w.rho[b]
-        if e / bound > worst_ratio[0]:
-            worst_ratio = (e / bound, b, e, bound)
+        if prec == "fp16":
+            bound = 4 * U * np.sqrt(float(w.K[b])) * w.R[b]
+        else:
+            bound = 4 * 2.0 ** -13 * w.sq[b]
+        if e / bound > worst[0]:
+            worst = (e / bound, b, e, bound)
         assert e <= bound, (prec, int(b), int(w.M[b]), int(w.N[b]), int(w.K[b]), e, bound)
-        ref = ALPHA * w.exact[b] + BETA * w.c0[b]
-        e64 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        e64 = float(np.linalg.norm(got - w.ref[b]) / np.linalg.norm(w.ref[b]))
         worst_fp64 = max(worst_fp64, e64)
         assert e64 <= {"fp16": 2e-3, "fp8": 0.2}[prec], (prec, int(b), e64)
-    r, b, e, bound = worst_ratio
-    say(f"  {prec} pack={pack}: worst err/bound {r:.3g} vs emulation (matrix {b}: "
-        f"{w.M[b]}x{w.N[b]}x{w.K[b]}, err {e:.3g}, bound {bound:.3g}); max vs FP64 {worst_fp64:.3g}")
+    r, b, e, bound = worst
+    say(f"  {prec} pack={pack}: vs emulation, worst matrix {b} ({w.M[b]}x{w.N[b]}x{w.K[b]}) err {e:.3g} "
+        f"= {r:.3g} of its bound {bound:.3g}; max vs FP64 {worst_fp64:.3g}")
 
 
 def test_batching_workload_bf16x3_vs_fp64(bg, gpu):
@@ -163,7 +186,7 @@ def test_batching_workload_bf16x3_vs_fp64(bg, gpu):
     plan.run(vb.dA, vb.dB, vb.dC, ALPHA, BETA)
     torch.cuda.synchronize()
     Ah, Bh, Ch, C0h = vb.A.cpu().numpy(), vb.B.cpu().numpy(), vb.C.cpu().numpy(), C0.cpu().numpy()
-    worst = 0.0
+    num = den = worst = 0.0
     for b in range(len(M)):
         m, n, k = int(M[b]), int(N[b]), int(K[b])
         A = Ah[vb.offA[b]: vb.offA[b] + m * k].reshape(k, m).T.astype(np.float64)
@@ -171,7 +194,12 @@ def test_batching_workload_bf16x3_vs_fp64(bg, gpu):
         c0 = C0h[vb.offC[b]: vb.offC[b] + m * n].reshape(n, m).T.astype(np.float64)
         got = Ch[vb.offC[b]: vb.offC[b] + m * n].reshape(n, m).T.astype(np.float64)
         ref = ALPHA * (A @ B) + BETA * c0
-        e = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-        worst = max(worst, e)
-        assert e <= 1e-5, (b, m, n, k, e)
-    say(f"  bf16x3 batching batch: max rel err vs FP64 {worst:.3g} over {len(M)} matrices")
+        d, nr = np.linalg.norm(got - ref), np.linalg.norm(ref)
+        num += d * d; den += nr * nr
+        R = ALPHA * np.linalg.norm(np.abs(A) @ np.abs(B)) / nr
+        worst = max(worst, d / nr)
+        assert d / nr <= bf16x3_bound(k, R), (b, m, n, k, d / nr)
+    pooled = (num / den) ** 0.5
+    say(f"  bf16x3 batching batch: pooled rel err vs FP64 {pooled:.3g}, worst matrix {worst:.3g} "
+        f"over {len(M)} matrices")
+    assert pooled <= 1e-5
