@@ -36,6 +36,7 @@ EXPORTS = [
     "vbg_plan_create", "vbg_plan_destroy", "vbg_sgemm_vbatched", "vbg_sgemm_vbatched_oneshot",
     "vbg_plan_query", "vbg_ref_plan", "vbg_grouped_plan_create", "vbg_grouped_plan_destroy",
     "vbg_grouped_gemm_fwd", "vbg_grouped_gemm_bwd", "vbg_plan_launch_times",
+    "vbg_sgemm_vbatched_noalloc",
 ]
 
 
@@ -96,6 +97,7 @@ def lib():
     L.vbg_sgemm_vbatched_oneshot.argtypes = [ctypes.c_int, ip, ip, ip, ip, ip, ip, ctypes.c_char,
                                              ctypes.c_char, ctypes.c_int, ctypes.c_float, vp, vp,
                                              ctypes.c_float, vp, vp]
+    L.vbg_sgemm_vbatched_noalloc.argtypes = L.vbg_sgemm_vbatched_oneshot.argtypes
     L.vbg_plan_query.argtypes = [vp, ctypes.c_int, ip, ctypes.c_int]
     L.vbg_plan_launch_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
     L.vbg_ref_plan.argtypes = [ctypes.c_int, ip, ip, ip, ctypes.c_int, ip, ip, ip,

@@ -12,6 +12,7 @@
 #include "../../include/vbgemm.h"
 #include "vbg_launch.h"
 #include "vbg_planner.h"
+#include "vbg_run.h"
 
 namespace {
 
@@ -27,43 +28,19 @@ int hip_fail(hipError_t e, const char* what) {
   return VBG_ERR_HIP;
 }
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-int occupancy_cb(int family, int prec, int tile, bool ta, bool tb) {
-  switch (family) {
-    case vbg::FAM_FP32: return vbg_occupancy_fp32(tile, ta, tb);
-    case vbg::FAM_SEQ: return vbg_occupancy_seq(ta, tb);
-    case vbg::FAM_SPLIT: return vbg_occupancy_split(prec, tile, ta, tb);
-    case vbg::FAM_SPLIT_PK: return vbg_occupancy_split_pk(prec, tile);
-    case vbg::FAM_BF16: return vbg_occupancy_bf16(prec, tile, ta, tb);
-    default: return 1;
-  }
-}
-
 }  // namespace
+
+void vbg::set_last_error(const std::string& msg) { g_err = msg; }
 
 struct vbg_plan {
   vbg::HostPlan hp;
   vbg_opts opts;
   int device = -1;
   bool host_only = false;
-  void* dbuf = nullptr;  // one device allocation: descs, items, queues, counters, scales
-  vbg_mat_desc* d_desc = nullptr;
-  float* d_scales = nullptr;  // [4 * batch] (FP16 / FP8 paths)
-  unsigned* d_amax = nullptr; // [2 * batch] float bits of per-matrix amax
-  vbg_chunk* d_chunks = nullptr;
-  vbg_chunk* d_red = nullptr;     // split-K reduction chunks
-  int64_t* d_ws_off = nullptr;    // per-matrix workspace offsets (floats)
-  float* d_ws = nullptr;          // split-K partial panels (own allocation)
-  char* d_pk = nullptr;           // packed operands (own allocation)
-  int64_t* d_pk_off = nullptr;    // [2 * batch] byte offsets into d_pk
-  vbg_pack_job* d_jobs = nullptr; // pack pre-pass jobs
-  struct DevLaunch {
-    vbg_item* items;
-    int* q_begin;
-    unsigned* counters;
-  };
-  std::vector<DevLaunch> dl;
+  void* dbuf = nullptr;  // one device allocation: the plan's image (descs, items, queues, counters, scales)
+  float* d_ws = nullptr; // split-K partial panels (own allocation)
+  char* d_pk = nullptr;  // packed operands (own allocation)
+  vbg::DeviceView view;
   // Kernel timing (opts.timing_slots > 0): `slots` rings of rows + 1 events.
   std::vector<hipEvent_t> ev;
   int slots = 0;
@@ -78,12 +55,7 @@ bool has_prepass(const vbg::HostPlan& hp) {
 
 bool has_pack(const vbg::HostPlan& hp) { return !hp.pack_jobs.empty(); }
 
-// Kernels a run issues, in order: [scale pre-pass], [operand pack], one per
-// launch, [split-K reduction].
-int kernel_rows(const vbg::HostPlan& hp) {
-  return (has_prepass(hp) ? 1 : 0) + (has_pack(hp) ? 1 : 0) + (int)hp.launches.size() +
-         (hp.red_chunks.empty() ? 0 : 1);
-}
+using vbg::kernel_rows;
 
 int family_kind(int fam) {
   switch (fam) {
@@ -176,95 +148,46 @@ int vbg_plan_create(vbg_plan** plan, int batch, const int* M, const int* N, cons
   p->host_only = host_only;
   std::string err;
   int rc = vbg::build_plan(p->hp, batch, M, N, K, lda, ldb, ldc, ta, tb, prec, o, cus,
-                           host_only ? nullptr : occupancy_cb, err);
+                           host_only ? nullptr : vbg::occupancy, err);
   if (rc != VBG_OK) {
     delete p;
     return fail(rc, err);
   }
   if (!host_only) {
     const vbg::HostPlan& hp = p->hp;
-    // layout
-    size_t off = 0;
-    const size_t off_desc = off;
-    off = align256(off + sizeof(vbg_mat_desc) * std::max(1, hp.batch));
-    const size_t off_scales = off;
-    off = align256(off + sizeof(float) * 4 * std::max(1, hp.batch));
-    const size_t off_amax = off;
-    off = align256(off + sizeof(unsigned) * 2 * std::max(1, hp.batch));
-    const size_t off_chunks = off;
-    off = align256(off + sizeof(vbg_chunk) * hp.chunks.size());
-    const size_t off_red = off;
-    off = align256(off + sizeof(vbg_chunk) * hp.red_chunks.size());
-    const size_t off_wsoff = off;
-    off = align256(off + sizeof(int64_t) * std::max(1, hp.batch));
-    const size_t off_pkoff = off;
-    off = align256(off + sizeof(int64_t) * hp.pk_off.size());
-    const size_t off_jobs = off;
-    off = align256(off + sizeof(vbg_pack_job) * hp.pack_jobs.size());
-    std::vector<size_t> oi, oq, oc;
-    for (const auto& L : hp.launches) {
-      oi.push_back(off);
-      off = align256(off + sizeof(vbg_item) * L.items.size());
-      oq.push_back(off);
-      off = align256(off + sizeof(int) * L.q_begin.size());
-      oc.push_back(off);
-      off = align256(off + sizeof(unsigned) * (hp.nq + 1));
-    }
-    std::vector<char> host(off, 0);
-    if (hp.batch) std::memcpy(host.data() + off_desc, hp.desc.data(), sizeof(vbg_mat_desc) * hp.batch);
-    if (!hp.chunks.empty())
-      std::memcpy(host.data() + off_chunks, hp.chunks.data(), sizeof(vbg_chunk) * hp.chunks.size());
-    if (!hp.red_chunks.empty())
-      std::memcpy(host.data() + off_red, hp.red_chunks.data(), sizeof(vbg_chunk) * hp.red_chunks.size());
-    if (hp.batch) std::memcpy(host.data() + off_wsoff, hp.ws_off.data(), sizeof(int64_t) * hp.batch);
-    if (!hp.pk_off.empty())
-      std::memcpy(host.data() + off_pkoff, hp.pk_off.data(), sizeof(int64_t) * hp.pk_off.size());
-    if (!hp.pack_jobs.empty())
-      std::memcpy(host.data() + off_jobs, hp.pack_jobs.data(), sizeof(vbg_pack_job) * hp.pack_jobs.size());
-    for (size_t l = 0; l < hp.launches.size(); ++l) {
-      const auto& L = hp.launches[l];
-      std::memcpy(host.data() + oi[l], L.items.data(), sizeof(vbg_item) * L.items.size());
-      std::memcpy(host.data() + oq[l], L.q_begin.data(), sizeof(int) * L.q_begin.size());
-    }
-    hipError_t e = hipMalloc(&p->dbuf, off);
+    vbg::ImageLayout lay;
+    std::vector<char> host;
+    vbg::build_image(hp, lay, host);
+    hipError_t e = hipMalloc(&p->dbuf, lay.bytes);
     if (e != hipSuccess) {
       delete p;
       return hip_fail(e, "hipMalloc(plan)");
     }
-    e = hipMemcpy(p->dbuf, host.data(), off, hipMemcpyHostToDevice);
+    e = hipMemcpy(p->dbuf, host.data(), lay.bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       (void)hipFree(p->dbuf);
       delete p;
       return hip_fail(e, "hipMemcpy(plan)");
     }
-    char* base = (char*)p->dbuf;
-    p->d_desc = (vbg_mat_desc*)(base + off_desc);
-    p->d_scales = (float*)(base + off_scales);
-    p->d_amax = (unsigned*)(base + off_amax);
-    p->d_chunks = (vbg_chunk*)(base + off_chunks);
-    p->d_red = (vbg_chunk*)(base + off_red);
-    p->d_ws_off = (int64_t*)(base + off_wsoff);
-    p->d_pk_off = (int64_t*)(base + off_pkoff);
-    p->d_jobs = (vbg_pack_job*)(base + off_jobs);
+    vbg::view_image(hp, lay, (char*)p->dbuf, p->view);
     if (hp.ws_floats > 0) {
       e = hipMalloc(&p->d_ws, sizeof(float) * hp.ws_floats);
       if (e != hipSuccess) {
-        (void)hipFree(p->dbuf);
-        delete p;
-        return hip_fail(e, "hipMalloc(split-K workspace)");
+        int rc2 = hip_fail(e, "hipMalloc(split-K workspace)");
+        vbg_plan_destroy(p);
+        return rc2;
       }
     }
     if (hp.pk_bytes > 0) {
       e = hipMalloc(&p->d_pk, (size_t)hp.pk_bytes);
       if (e != hipSuccess) {
         int rc2 = hip_fail(e, "hipMalloc(packed operands)");
-        p->d_pk = nullptr;
         vbg_plan_destroy(p);
         return rc2;
       }
     }
-    for (size_t l = 0; l < hp.launches.size(); ++l)
-      p->dl.push_back({(vbg_item*)(base + oi[l]), (int*)(base + oq[l]), (unsigned*)(base + oc[l])});
+    p->view.ws = p->d_ws;
+    p->view.pk = p->d_pk;
     if (o.timing_slots > 0) {
       p->slots = o.timing_slots;
       p->ev.assign((size_t)p->slots * (kernel_rows(hp) + 1), nullptr);
@@ -317,7 +240,6 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
 
   // Timing plans bracket every kernel with events of this run's ring slot.
   hipEvent_t* ev = nullptr;
-  int row = 0;
   if (p->slots > 0) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     e = hipStreamIsCapturing(s, &cs);
@@ -325,68 +247,10 @@ int vbg_sgemm_vbatched(vbg_plan* p, float alpha, const float* const* dA, const f
     if (cs != hipStreamCaptureStatusNone)
       return fail(VBG_ERR_NOT_SUPPORTED, "a timing plan cannot be captured in a graph");
     ev = &p->ev[(size_t)(p->runs % p->slots) * (kernel_rows(hp) + 1)];
-    e = hipEventRecord(ev[0], s);
-    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
   }
-  auto mark = [&]() -> int {
-    if (!ev) return VBG_OK;
-    hipError_t er = hipEventRecord(ev[++row], s);
-    return er == hipSuccess ? VBG_OK : hip_fail(er, "hipEventRecord");
-  };
-
-  if (has_prepass(hp)) {
-    int r = vbg_launch_scales(hp.prec, hp.batch, hp.ta, hp.tb, p->d_desc, p->d_chunks,
-                              (int)hp.chunks.size(), dA, dB, p->d_amax, p->d_scales, s);
-    if (r) return fail(VBG_ERR_HIP, std::string("scale pre-pass launch: ") + hipGetErrorString(hipGetLastError()));
-    if (int rc = mark()) return rc;
-  }
-  if (has_pack(hp)) {
-    int r = vbg_launch_pack(hp.prec, hp.ta, hp.tb, p->d_jobs, (int)hp.pack_jobs.size(), p->d_desc, dA, dB,
-                            p->d_scales, p->d_pk, p->d_pk_off, s);
-    if (r) return fail(VBG_ERR_HIP, std::string("operand pack launch: ") + hipGetErrorString(hipGetLastError()));
-    if (int rc = mark()) return rc;
-  }
-  for (size_t l = 0; l < hp.launches.size(); ++l) {
-    const vbg::Launch& L = hp.launches[l];
-    vbg_args a;
-    a.desc = p->d_desc;
-    a.items = p->dl[l].items;
-    a.q_begin = p->dl[l].q_begin;
-    a.counters = p->dl[l].counters;
-    a.A = dA;
-    a.B = dB;
-    a.C = dC;
-    a.ws = p->d_ws;
-    a.ws_off = p->d_ws_off;
-    a.scales = p->d_scales;
-    a.pk = L.family == vbg::FAM_SPLIT_PK ? p->d_pk : nullptr;
-    a.pk_off = p->d_pk_off;
-    a.alpha = alpha;
-    a.beta = beta;
-    a.nq = hp.nq;
-    a.trans_a = hp.ta;
-    a.trans_b = hp.tb;
-    a.prec = hp.prec;
-    int r;
-    switch (L.family) {
-      case vbg::FAM_FP32: r = vbg_launch_fp32(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
-      case vbg::FAM_SEQ: r = vbg_launch_seq(hp.ta, hp.tb, &a, L.grid, s); break;
-      case vbg::FAM_SPLIT: r = vbg_launch_split(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
-      case vbg::FAM_SPLIT_PK: r = vbg_launch_split_pk(L.tile, &a, L.grid, s); break;
-      case vbg::FAM_BF16: r = vbg_launch_bf16(L.tile, hp.ta, hp.tb, &a, L.grid, s); break;
-      default: r = -1;
-    }
-    if (r == -1) return fail(VBG_ERR_NOT_SUPPORTED, "tile/precision combination not built");
-    if (r != 0) return fail(VBG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(hipGetLastError()));
-    if (int rc = mark()) return rc;
-  }
-  if (!hp.red_chunks.empty()) {
-    int r = vbg_launch_splitk_reduce(p->d_red, (int)hp.red_chunks.size(), p->d_desc, p->d_ws,
-                                     p->d_ws_off, dC, alpha, beta,
-                                     hp.prec == VBG_PREC_BF16_OUT_BF16, s);
-    if (r != 0) return fail(VBG_ERR_HIP, std::string("split-K reduction launch: ") + hipGetErrorString(hipGetLastError()));
-    if (int rc = mark()) return rc;
-  }
+  std::string err;
+  const int rc = vbg::issue_run(hp, p->view, alpha, dA, dB, beta, dC, s, ev, err);
+  if (rc != VBG_OK) return fail(rc, err);
   if (ev) ++p->runs;
   return VBG_OK;
 }

@@ -267,7 +267,8 @@ int ref_plan(int batch, const int* M, const int* N, const int* K, int thres, int
 
 int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K, const int* lda,
                const int* ldb, const int* ldc, bool ta, bool tb, int prec, const vbg_opts& o,
-               int num_cus, int (*occupancy)(int, int, int, bool, bool), std::string& err) {
+               int num_cus, int (*occupancy)(int, int, int, bool, bool), std::string& err,
+               int64_t ws_cap_floats) {
   hp = HostPlan();
   hp.batch = batch;
   hp.ta = ta;
@@ -471,7 +472,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     for (int b = 0; b < batch; ++b)
       if (M[b] > 0 && N[b] > 0) total += matrix_cost(tile[b], sh[b]);
     const double avg = total / ((double)cus * occ(fam, T128));
-    const int64_t ws_cap = (int64_t)1 << 27;  // floats (512 MB)
+    const int64_t ws_cap = ws_cap_floats;
     int64_t ws = 0;
     for (int b = 0; b < batch; ++b) {
       if (M[b] == 0 || N[b] == 0) continue;

@@ -47,11 +47,14 @@ struct HostPlan {
 
 // Fills `hp` from the shapes and options.  `occupancy(tile)` gives the
 // resident workgroups per CU for the family that runs `tile` (1 if unknown).
-// Returns 0 or a negative vbg_status, with `err` set.
+// Split-K partial panels take at most `ws_cap_floats` floats of workspace in
+// total (matrices that would not fit are not split).  Returns 0 or a negative
+// vbg_status, with `err` set.
+constexpr int64_t kDefaultWsCapFloats = (int64_t)1 << 27;  // 512 MB
 int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K, const int* lda,
                const int* ldb, const int* ldc, bool ta, bool tb, int prec, const vbg_opts& o,
                int num_cus, int (*occupancy)(int family, int prec, int tile, bool ta, bool tb),
-               std::string& err);
+               std::string& err, int64_t ws_cap_floats = kDefaultWsCapFloats);
 
 // Reference planner restated from ROCm/framework/gemm.cuh:121-229.
 int ref_plan(int batch, const int* M, const int* N, const int* K, int thres, int* t_strategy,

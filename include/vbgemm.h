@@ -168,6 +168,30 @@ int vbg_sgemm_vbatched_oneshot(int batch, const int* M, const int* N, const int*
                                const float* const* dB, float beta, float* const* dC,
                                void* stream);
 
+/*
+ * One-shot form that allocates nothing: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code that must not call
+ * hipMalloc: serving loops, harnesses that forbid allocation).  Same semantics
+ * as vbg_sgemm_vbatched_oneshot for FP32-operand paths (prec 0-4; BF16
+ * operands return VBG_ERR_NOT_SUPPORTED), but the plan's device work map and
+ * split-K workspace live in a static device array of
+ * VBG_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until the
+ * process ends: at most VBG_NOALLOC_IMAGE_BYTES of work map (a batch that
+ * needs more returns VBG_ERR_INVALID_ARG; about 16 bytes per work item and 80
+ * per matrix), the rest split-K partial panels (matrices whose panels would
+ * not fit are not split).  Split precisions quantise in the k-loop (no packed
+ * operands).  Uploads with hipMemcpyAsync and synchronises `stream` before it
+ * returns.  Every call on one GPU shares that GPU's array: calls that overlap
+ * in time on one GPU (other streams or threads) race.
+ */
+#define VBG_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define VBG_NOALLOC_IMAGE_BYTES (16u << 20)
+int vbg_sgemm_vbatched_noalloc(int batch, const int* M, const int* N, const int* K,
+                               const int* lda, const int* ldb, const int* ldc, char transA,
+                               char transB, int prec, float alpha, const float* const* dA,
+                               const float* const* dB, float beta, float* const* dC,
+                               void* stream);
+
 /* Plan inspection (works on host_only plans).  what:
  *   0 number of work items             1 number of kernel launches (families used)
  *   2 tile id per matrix (out[batch])  3 persistent grid size of launch 0
