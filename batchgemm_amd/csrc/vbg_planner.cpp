@@ -31,11 +31,14 @@ namespace vbg {
 namespace {
 
 // k-tile depth of each tile id (matches the kernels).
-const int kTileBK[NTILES] = {64, 32, 32, 32, 32, 32};
+const int kTileBK[NTILES] = {64, 32, 32, 32, 32, 32, 16};
 
 // Relative MFMA efficiency of each tile shape on gfx950 (per-flop cost factor
 // = 1/eff).  Larger tiles reuse each LDS byte and each HBM byte more times.
-const double kTileEff[NTILES] = {0.20, 0.40, 0.70, 0.85, 0.85, 1.00};
+// 256x128 (bf16x3 only): 1.10, from 1024^3 bf16x3 batches at 311 vs 283
+// TFLOP/s against 128x128 (two k-tiles of 16 against one of 32 per barrier,
+// 25% fewer operand bytes per flop).
+const double kTileEff[NTILES] = {0.20, 0.40, 0.70, 0.85, 0.85, 1.00, 1.10};
 // Fixed cost of one work item in units of a 128x128x32 k-step (queue pop,
 // pipeline fill, epilogue).
 const double kItemFixed = 3.0;
@@ -65,12 +68,21 @@ bool is_split(int prec) {
 }
 
 // The split-precision and BF16 families are built on 32x32 MFMA blocks only
-// (>= 64x64).
+// (>= 64x64); the 256x128 tile for bf16x3 only, and AUTO uses it only when
+// the tile mask names it: with it, the unbalanced B=2048 bf16x3 batch ran
+// 3.700 vs 3.714 ms (the 21 large matrices it leaves on 128x128 tiles run as
+// a second launch after it), and batches of the batching workload 4-11%
+// slower (two launches in series; one combined kernel spilled registers and
+// was no faster).  On one shape alone it gains 5-10% (SplitTile notes).
 bool tile_allowed(int prec, int tile, unsigned mask) {
   if (mask && !(mask & (1u << tile))) return false;
   if (is_split(prec) && tile < T64) return false;
+  if (tile == T256x128 && (prec != VBG_PREC_BF16X3 || !(mask & (1u << T256x128)))) return false;
   return true;
 }
+
+// Shapes a mixed launch holds (one kernel dispatching on each item's shape).
+bool mixable(int tile) { return tile >= T64 && tile <= T128; }
 
 // AUTO: per-matrix minimum modelled cost, then, while the batch exposes fewer
 // work items than resident workgroup slots, split the matrix whose single
@@ -105,7 +117,7 @@ void choose_auto(const std::vector<Shape>& sh, int prec, const vbg_opts& o, int 
   const bool split_ok = prec != VBG_PREC_FP32_SEQ && o.split_k_min >= 0;
   int64_t items = 0;
   for (int b = 0; b < B; ++b) items += num_tiles(tile[b], sh[b]);
-  const int smaller[NTILES] = {-1, -1, -1, T64, T64, T128x64};
+  const int smaller[NTILES] = {-1, -1, -1, T64, T64, T128x64, T128};
   for (int iter = 0; iter < 4 * B && items < slots; ++iter) {
     int pick = -1;
     double longest = 0;
@@ -196,14 +208,15 @@ void choose_wang(const std::vector<Shape>& sh, int thres, std::vector<int>& tile
 // tm == M and tn <= N, else 16x16.
 void choose_li(const std::vector<Shape>& sh, int thres, std::vector<int>& tile) {
   const int B = (int)sh.size();
+  const int kLiTiles = T128 + 1;  // Li's six tiles, 16x16 .. 128x128
   int t;
-  for (t = 0; t < NTILES; ++t) {
+  for (t = 0; t < kLiTiles; ++t) {
     int64_t tlp = 0;
     for (int j = 0; j < B; ++j)
       tlp += (int64_t)(sh[j].M / kTileBM[t]) * (sh[j].N / kTileBN[t]) * 256;
     if (tlp < thres) break;
   }
-  if (t == NTILES) t = NTILES - 1;
+  if (t == kLiTiles) t = kLiTiles - 1;
   for (int j = 0; j < B; ++j) {
     int s = T16;
     if (kTileBM[t] <= sh[j].M && kTileBN[t] <= sh[j].N) {
@@ -341,6 +354,10 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
           err = "fixed_tile out of range";
           return VBG_ERR_INVALID_ARG;
         }
+        if (o.fixed_tile == T256x128 && prec != VBG_PREC_BF16X3) {
+          err = "the 256x128 tile is built for bf16x3 only";
+          return VBG_ERR_NOT_SUPPORTED;
+        }
         std::fill(tile.begin(), tile.end(), o.fixed_tile);
         break;
       case VBG_POLICY_REFERENCE: {
@@ -419,7 +436,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   // instead of four (220 -> 223 TFLOP/s).
   if (o.policy == VBG_POLICY_AUTO && o.consolidate >= 0 && fam != FAM_SEQ && batch > 0) {
     const double kLaunchFixed = 8.0, kAbsorbFrac = 0.05;
-    double tot[NTILES] = {0, 0, 0, 0, 0, 0}, longest[NTILES] = {0, 0, 0, 0, 0, 0};
+    double tot[NTILES] = {}, longest[NTILES] = {};
     for (int b = 0; b < batch; ++b) {
       if (M[b] == 0 || N[b] == 0) continue;
       tot[tile[b]] += matrix_cost(tile[b], sh[b]);
@@ -528,7 +545,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   bool small_batch = false;
   {
     const double kSmallBatch = 32.0;
-    double tot[NTILES] = {0, 0, 0, 0, 0, 0}, longest[NTILES] = {0, 0, 0, 0, 0, 0};
+    double tot[NTILES] = {}, longest[NTILES] = {};
     for (int b = 0; b < batch; ++b) {
       if (M[b] == 0 || N[b] == 0) continue;
       tot[tile[b]] += matrix_cost(tile[b], sh[b]);
@@ -557,9 +574,10 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
       const int r = M[b] % BMt, s = N[b] % BNt;
       const int t_bot = BNt == 128 ? T64x128 : T64;  // 64 x BN
       const int t_rgt = BMt == 128 ? T128x64 : T64;  // BM x 64
-      const bool half_m = edges && BMt == 128 && r > 0 && r <= 64 && tile_allowed(prec, t_bot, o.tile_mask) &&
+      // (edge strips only for tiles with both sides <= 128)
+      const bool half_m = edges && BMt == 128 && BNt <= 128 && r > 0 && r <= 64 && tile_allowed(prec, t_bot, o.tile_mask) &&
                           (BNt == 128 || tile_allowed(prec, T64, o.tile_mask));
-      const bool half_n = edges && BNt == 128 && s > 0 && s <= 64 && tile_allowed(prec, t_rgt, o.tile_mask) &&
+      const bool half_n = edges && BNt == 128 && BMt <= 128 && s > 0 && s <= 64 && tile_allowed(prec, t_rgt, o.tile_mask) &&
                           (BMt == 128 || tile_allowed(prec, T64, o.tile_mask));
       const int rows = half_m ? M[b] / BMt : cdiv(M[b], BMt);
       const int cols = half_n ? N[b] / BNt : cdiv(N[b], BNt);
@@ -573,12 +591,12 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   if (mix_family && o.mixed_tiles == 0 && !mixed) {
     const double kMixFrac = 0.12;
     build_parts(o.edge_tiles >= 0);  // the parts a mixed launch would have
-    double cost[NTILES] = {0, 0, 0, 0, 0, 0};
-    for (int t = T64; t < NTILES; ++t)
+    double cost[NTILES] = {};
+    for (int t = T64; t <= T128; ++t)
       for (const Part& x : parts[t])
         cost[t] += (double)(x.tm1 - x.tm0) * (x.tn1 - x.tn0) * item_cost(t, sh[x.b]);
     double all = 0.0, top = 0.0;
-    for (int t = T64; t < NTILES; ++t) {
+    for (int t = T64; t <= T128; ++t) {
       all += cost[t];
       top = std::max(top, cost[t]);
     }
@@ -609,7 +627,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   if (mix_family && !mixed && o.mixed_tiles == 0 && VBG_PARTIAL_MIX) {
     double best = 0.0;
     int others = 0;
-    for (int t = T64; t < NTILES; ++t) {
+    for (int t = T64; t <= T128; ++t) {
       if (parts[t].empty()) continue;
       double c = 0.0;
       for (const Part& x : parts[t]) c += (double)(x.tm1 - x.tm0) * (x.tn1 - x.tn0) * item_cost(t, sh[x.b]);
@@ -625,7 +643,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     std::vector<TPart> mix;
     for (int t = 0; t < NTILES; ++t) {
       if (parts[t].empty()) continue;
-      const bool into_mix = (mixed && t >= T64) || (keep >= 0 && t >= T64 && t != keep);
+      const bool into_mix = (mixed && mixable(t)) || (keep >= 0 && mixable(t) && t != keep);
       std::vector<TPart> g;
       for (const Part& x : parts[t]) {
         const double item = slice_cost(t, x.b);

@@ -76,7 +76,8 @@ typedef enum vbg_tile {
   VBG_TILE_128x64 = 3,
   VBG_TILE_64x128 = 4,
   VBG_TILE_128x128 = 5,
-  VBG_TILE_COUNT = 6
+  VBG_TILE_256x128 = 6,     /* bf16x3 only: 4 waves of 128 x 64 each, 16-deep k-tiles */
+  VBG_TILE_COUNT = 7
 } vbg_tile;
 
 typedef struct vbg_opts {
@@ -84,7 +85,8 @@ typedef struct vbg_opts {
   int wavefront_thres;  /* REFERENCE: cascade threshold (reference default 6000) */
   int tlp_thres;        /* WANG / LI: TLP threshold */
   int fixed_tile;       /* FIXED: vbg_tile id */
-  unsigned tile_mask;   /* AUTO: bit t enables tile t (0 = all) */
+  unsigned tile_mask;   /* AUTO: bit t enables tile t (0 = all but 256x128, which AUTO uses
+                           only when its bit is set) */
   int big_tile_min_mn;  /* AUTO: min(M,N) at which 128x128 becomes eligible (0 = default) */
   int split_k_min;      /* split K of matrices with K >= this whose one item would be the
                            launch's critical path (0 = 256, <0 = never split) */
@@ -200,7 +202,7 @@ int vbg_sgemm_vbatched_noalloc(int batch, const int* M, const int* N, const int*
  *   6 work items as (mat, tile id, tile_m, tile_n, k slice) x n (out[5*n]); tile_m / tile_n
  *     count in units of that item's tile shape
  *   7 kernel table, one row of (kind, tile id, first item, item count, grid) per kernel a run
- *     issues, in issue order (out[5*rows]); tile id 6 = a mixed launch (items of the 64x64 ..
+ *     issues, in issue order (out[5*rows]); tile id 7 = a mixed launch (items of the 64x64 ..
  *     128x128 shapes, see query 6); kind 0 = scale pre-pass (FP16 / FP8), 1 = FP32
  *     MFMA tiles, 2 = FP32 sequential-fma tiles, 3 = split-precision tiles, 4 = BF16 tiles,
  *     5 = split-K reduction, 6 = operand pack pre-pass (grid = pack jobs), 7 = split-precision

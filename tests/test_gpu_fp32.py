@@ -65,7 +65,7 @@ def test_fp32_every_tile_every_transpose(bg, orc, gpu, tile, trans, mixed):
     vb, C0, plan = run_case(bg, orc, M, N, K, trans[0], trans[1], 1.5, 0.5, 1, policy="fixed",
                             fixed_tile=tile, mixed_tiles=mixed)
     kt = plan.kernels()
-    assert (kt[:, 1] == 6).any() == (mixed > 0 and tile not in ("16x16", "32x32"))
+    assert (kt[:, 1] == bg._lib.TILE_MIXED).any() == (mixed > 0 and tile not in ("16x16", "32x32"))
     oracle_check(orc, vb, 1.5, 0.5, C0)
 
 

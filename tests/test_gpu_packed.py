@@ -10,8 +10,10 @@ test_gpu_precision.py.  Cases: every transpose pair, ragged M / N / K (K not a
 multiple of the 64-element row padding, K = 0, M or N = 1), several pack jobs
 per operand (M, N > 64 and K > 256), leading dimensions past the row count
 whose padding holds 1e30 (a pack that read it would blow the result up),
-split-K, mixed and per-shape launches, a second run on new operand values, and
-graph replays."""
+split-K, mixed and per-shape launches, every tile shape on its own (the
+packed kernels stage each shape's k-tiles through LDS in 1 KB pieces whose
+split between the operands depends on the shape), a second run on new operand
+values, and graph replays."""
 import sys
 
 import numpy as np
@@ -48,7 +50,9 @@ def test_packed_equals_unpacked(bg, gpu, prec, trans):
     import torch
     from batchgemm_amd.workloads import VBatch
     M, N, K = (np.asarray(x, np.int32) for x in SHAPES)
-    for opts in ({"mixed_tiles": -1}, {"mixed_tiles": 1, "force_ksplit": 3}):
+    fixed = [{"policy": "fixed", "fixed_tile": t, "mixed_tiles": -1}
+             for t in ("64x64", "128x64", "64x128", "128x128") + (("256x128",) if prec == "bf16x3" else ())]
+    for opts in ({"mixed_tiles": -1}, {"mixed_tiles": 1, "force_ksplit": 3}, *fixed):
         say(f"[packed] {prec} trans={trans} {opts}")
         vb = VBatch(M, N, K, trans[0], trans[1], pad_ab=3)
         plans = {pk: bg.Plan(M, N, K, vb.lda, vb.ldb, vb.ldc, trans[0], trans[1], prec=prec,

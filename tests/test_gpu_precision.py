@@ -174,14 +174,18 @@ def test_mixed_launch_equals_per_shape_launches(bg, gpu, prec):
     vb = w.VBatch(M, N, K).fill("normal", seed=31, outlier_frac=0.01, c_dist="uniform")
     C0 = vb.C.clone()
     outs = []
+    # split-K off in both: the two launch forms have different workgroup slot
+    # counts (occupancy per tile shape), so the planner may split K differently,
+    # which changes the summation order
     for opts in ({"mixed_tiles": 1}, {"mixed_tiles": -1, "edge_tiles": -1}):
         vb.C.copy_(C0)
-        p = bg.Plan(M, N, K, prec=prec, **opts)
+        p = bg.Plan(M, N, K, prec=prec, split_k_min=-1, **opts)
         p.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
         torch.cuda.synchronize()
         outs.append((vb.C.clone(), p.kernels()))
     (c0, k0), (c1, k1) = outs
-    assert (k0[:, 1] == 6).any() and not (k1[:, 1] == 6).any()
+    mixed_id = bg._lib.TILE_MIXED
+    assert (k0[:, 1] == mixed_id).any() and not (k1[:, 1] == mixed_id).any()
     assert torch.equal(c0.view(torch.int32), c1.view(torch.int32))
 
 
@@ -240,3 +244,45 @@ def test_scaled_paths_tiny_magnitudes(bg, orc, gpu, prec):
         emul = emulate(orc, prec, A, B)
         assert orc.rel_fro(got, emul) <= emul_bound(prec, A, B, emul), b
         assert orc.rel_fro(got, exact) <= FP64_BOUND[prec], b
+
+
+@pytest.mark.parametrize("trans", ["NN", "TN", "NT", "TT"])
+def test_bf16x3_256x128_tile(bg, orc, gpu, trans):
+    """The bf16x3 family's 256x128 tile (16-deep k-tiles, 128 x 64 per wave)
+    on ragged shapes -- rows past M in a partial 256-row tile, K not a
+    multiple of 16, K = 0 -- against the emulation and FP64, and bit-equal to
+    the 128x128 tile (the k grouping into MFMAs is the same)."""
+    import torch
+    from batchgemm_amd.workloads import VBatch
+    say(f"[bf16x3 256x128] trans={trans}")
+    M = [256, 300, 17, 130, 513, 64]
+    N = [128, 33, 140, 129, 250, 1]
+    K = [64, 200, 9, 257, 35, 0]
+    vb = VBatch(M, N, K, trans[0], trans[1]).fill("normal", seed=19, outlier_frac=0.01, c_dist="uniform")
+    C0 = vb.C.clone()
+    out = {}
+    for tile in ("256x128", "128x128"):
+        vb.C.copy_(C0)
+        p = bg.Plan(vb.M, vb.N, vb.K, vb.lda, vb.ldb, vb.ldc, trans[0], trans[1], prec="bf16x3",
+                    policy="fixed", fixed_tile=tile, mixed_tiles=-1, split_k_min=-1)
+        assert (p.tiles() == bg.TILES[tile]).all()
+        p.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
+        torch.cuda.synchronize()
+        out[tile] = vb.C.clone()
+    assert torch.equal(out["256x128"].view(torch.int32), out["128x128"].view(torch.int32))
+    C0h = C0.cpu().numpy()
+    vb.C.copy_(out["256x128"])
+    for b in range(vb.batch):
+        A = op_matrix(vb, "A", b); B = op_matrix(vb, "B", b)
+        c0 = C0h[int(vb.offC[b]): int(vb.offC[b]) + M[b] * N[b]].reshape(N[b], M[b]).T.astype(np.float64)
+        got = vb.host_flat("C", b).reshape(N[b], M[b]).T.astype(np.float64)
+        emul = 1.5 * emulate(orc, "bf16x3", A, B) + 0.5 * c0
+        exact = 1.5 * (A.astype(np.float64) @ B.astype(np.float64)) + 0.5 * c0
+        assert orc.rel_fro(got, emul) <= EMUL_BOUND["bf16x3"], b
+        assert orc.rel_fro(got, exact) <= FP64_BOUND["bf16x3"], b
+
+
+def test_256x128_tile_refused_outside_bf16x3(bg, gpu):
+    from batchgemm_amd._lib import VbgError
+    with pytest.raises(VbgError):
+        bg.Plan([300], [200], [100], prec="fp32", policy="fixed", fixed_tile="256x128")
