@@ -232,6 +232,99 @@ struct Fp32SmallTile {
   }
 };
 
+// Skinny FP32 tiles for matrices with one side of at most 16: 16 x 128 (M <=
+// 16) and 128 x 16 (N <= 16), on v_mfma_f32_16x16x4_f32.  Such a matrix on a
+// square tile computes mostly padding (an 8 x 1000 x 1000 product on 128x64
+// tiles issues 16x its useful MFMA work) and re-reads the short operand once
+// per tile of the long side; here one item covers 128 of the long side, the
+// short operand's 16 x 64 k-slab is loaded once per k-tile for all of it, and
+// each wave owns two 16x16 blocks along the long side.  Within a block the
+// 64-deep k-tile is spread over the lanes -- lane group g = l/16 takes k in
+// [16g, 16g + 16), as four float4 LDS reads -- and the MFMA's 4-slot sum adds
+// the lane groups' partial products, so the k reduction happens across lanes
+// inside the matrix core with no LDS round trip.
+template <bool WIDE, bool TA, bool TB>
+struct Fp32SkinnyTile {
+  static constexpr int BM = WIDE ? 16 : 128, BN = WIDE ? 128 : 16;
+  static constexpr int BK = 64;
+  static constexpr int P = BK + 4;
+  static constexpr int STAGE = (BM + BN) * P;
+  static constexpr int LDS_FLOATS = 2 * STAGE;
+  static constexpr int KL = BK / 4;  // k per lane group per k-tile
+  using LA = TileLoader<BM, BK, P, TA>;
+  using LB = TileLoader<BN, BK, P, !TB>;
+
+  __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
+    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
+    const float* __restrict__ A = vbg_uptr(a.A[it.mat]);
+    const float* __restrict__ B = vbg_uptr(a.B[it.mat]);
+    const int m0 = it.tm * BM, n0 = it.tn * BN;
+    const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
+    const int kb = kr.kb, ke = kr.ke;
+    const int nk = (ke - kb + BK - 1) / BK;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
+    // block j (0, 1) of this wave: rows / columns 32*wave + 16*j of the long side
+    v4f acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+    LA la;
+    LB lb;
+    la.init(A, d.lda, m0, d.M, d.K);
+    lb.init(B, d.ldb, n0, d.N, d.K);
+    typename LA::Regs ra;
+    typename LB::Regs rb;
+    if (nk > 0) {
+      la.fetch(ra, kb, ke);
+      lb.fetch(rb, kb, ke);
+      LA::store(ra, lds);
+      LB::store(rb, lds + BM * P);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) {
+        const int k0 = kb + (kt + 1) * BK;
+        la.fetch(ra, k0, ke);
+        lb.fetch(rb, k0, ke);
+      }
+      const float* As = lds + cur * STAGE;
+      const float* Bs = As + BM * P;
+#pragma unroll
+      for (int c = 0; c < KL / 4; ++c) {
+        const int ko = g * KL + 4 * c;
+        // the short side's fragment is shared by both blocks
+        const v4f sv = WIDE ? *(const v4f*)(As + l16 * P + ko) : *(const v4f*)(Bs + l16 * P + ko);
+        v4f lv[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int r = 32 * wave + 16 * j + l16;
+          lv[j] = WIDE ? *(const v4f*)(Bs + r * P + ko) : *(const v4f*)(As + r * P + ko);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[j] = WIDE ? __builtin_amdgcn_mfma_f32_16x16x4f32(sv[e], lv[j][e], acc[j], 0, 0, 0)
+                          : __builtin_amdgcn_mfma_f32_16x16x4f32(lv[j][e], sv[e], acc[j], 0, 0, 0);
+      }
+      if (more) {
+        float* nxt = lds + (cur ^ 1) * STAGE;
+        LA::store(ra, nxt);
+        LB::store(rb, nxt + BM * P);
+      }
+      __syncthreads();
+    }
+    // 16x16x4 layout: register r of lane l holds D[4*(l/16) + r][l%16]
+    const OutTile o = vbg_out(a, d, it, 1.0f);
+    auto seg = [&](int sg, int& row, int& col, v4f& v) {
+      row = m0 + (WIDE ? 0 : 32 * wave + 16 * sg) + 4 * g;
+      col = n0 + (WIDE ? 32 * wave + 16 * sg : 0) + l16;
+      v = acc[sg];
+    };
+    vbg_epilogue<2>(o, d.M, d.N, seg);
+  }
+};
+
 // Reference-compatible VALU tile (32x32 outputs, 4 per thread).  Each output
 // is one fmaf chain over k = 0, 1, ..., K-1 in order.  With alpha = beta = 1
 // the chain is seeded from C, which is exactly the reference's correct tiles
@@ -338,15 +431,19 @@ struct Fp32Mixed {
   using Tall = Fp32MfmaTile<128, 64, 32, TA, TB, false>;
   using Wide = Fp32MfmaTile<64, 128, 32, TA, TB, false>;
   using Sq64 = Fp32MfmaTile<64, 64, 32, TA, TB, false>;
-  static constexpr int LDS_FLOATS = Sq128::LDS_FLOATS;
+  using SkinnyM = Fp32SkinnyTile<true, TA, TB>;   // 16 x 128
+  using SkinnyN = Fp32SkinnyTile<false, TA, TB>;  // 128 x 16
+  static constexpr int LDS_FLOATS = SkinnyM::LDS_FLOATS > Sq128::LDS_FLOATS ? SkinnyM::LDS_FLOATS
+                                                                              : Sq128::LDS_FLOATS;
   static constexpr int MIN_WAVES = 2;  // two workgroups per CU, as the 128x128 body alone
-  static_assert(Sq128::LDS_FLOATS >= Tall::LDS_FLOATS && Sq128::LDS_FLOATS >= Sq64::LDS_FLOATS,
-                "the 128x128 body must need the most LDS");
+  static_assert(LDS_FLOATS >= Tall::LDS_FLOATS && LDS_FLOATS >= Sq64::LDS_FLOATS, "LDS sized for the largest body");
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
     switch (it.tile) {
       case T128: Sq128::run(a, it, lds); break;
       case T128x64: Tall::run(a, it, lds); break;
       case T64x128: Wide::run(a, it, lds); break;
+      case T16x128: SkinnyM::run(a, it, lds); break;
+      case T128x16: SkinnyN::run(a, it, lds); break;
       default: Sq64::run(a, it, lds); break;
     }
   }
@@ -397,6 +494,8 @@ template <bool TA, bool TB> using F32T128x64 = Fp32MfmaTile<128, 64, 32, TA, TB>
 template <bool TA, bool TB> using F32T64x128 = Fp32MfmaTile<64, 128, 32, TA, TB>;
 template <bool TA, bool TB> using F32T128 = Fp32MfmaTile<128, 128, 32, TA, TB>;
 template <bool TA, bool TB> using F32Mixed = Fp32Mixed<TA, TB>;
+template <bool TA, bool TB> using F32S16x128 = Fp32SkinnyTile<true, TA, TB>;
+template <bool TA, bool TB> using F32S128x16 = Fp32SkinnyTile<false, TA, TB>;
 
 static int fp32_dispatch(int tile, bool ta, bool tb, const vbg_args* a, int grid, hipStream_t s,
                          int* occ) {
@@ -407,6 +506,8 @@ static int fp32_dispatch(int tile, bool ta, bool tb, const vbg_args* a, int grid
     case T128x64: return dispatch_tt<F32T128x64>(ta, tb, a, grid, s, occ);
     case T64x128: return dispatch_tt<F32T64x128>(ta, tb, a, grid, s, occ);
     case T128: return dispatch_tt<F32T128>(ta, tb, a, grid, s, occ);
+    case T16x128: return dispatch_tt<F32S16x128>(ta, tb, a, grid, s, occ);
+    case T128x16: return dispatch_tt<F32S128x16>(ta, tb, a, grid, s, occ);
     case T_MIXED: return dispatch_tt<F32Mixed>(ta, tb, a, grid, s, occ);
     default: return -1;
   }

@@ -31,14 +31,16 @@ namespace vbg {
 namespace {
 
 // k-tile depth of each tile id (matches the kernels).
-const int kTileBK[NTILES] = {64, 32, 32, 32, 32, 32, 16};
+const int kTileBK[NTILES] = {64, 32, 32, 32, 32, 32, 16, 64, 64};
 
 // Relative MFMA efficiency of each tile shape on gfx950 (per-flop cost factor
 // = 1/eff).  Larger tiles reuse each LDS byte and each HBM byte more times.
 // 256x128 (bf16x3 only): 1.10, from 1024^3 bf16x3 batches at 311 vs 283
 // TFLOP/s against 128x128 (two k-tiles of 16 against one of 32 per barrier,
-// 25% fewer operand bytes per flop).
-const double kTileEff[NTILES] = {0.20, 0.40, 0.70, 0.85, 0.85, 1.00, 1.10};
+// 25% fewer operand bytes per flop).  The skinny 16x128 / 128x16 tiles
+// (FP32 only): 0.35, between the 16x16 and 32x32 tiles -- they stream the
+// long operand at an eighth of a 128x128 tile's MFMA work per byte.
+const double kTileEff[NTILES] = {0.20, 0.40, 0.70, 0.85, 0.85, 1.00, 1.10, 0.35, 0.35};
 // Fixed cost of one work item in units of a 128x128x32 k-step (queue pop,
 // pipeline fill, epilogue).
 const double kItemFixed = 3.0;
@@ -78,11 +80,13 @@ bool tile_allowed(int prec, int tile, unsigned mask) {
   if (mask && !(mask & (1u << tile))) return false;
   if (is_split(prec) && tile < T64) return false;
   if (tile == T256x128 && (prec != VBG_PREC_BF16X3 || !(mask & (1u << T256x128)))) return false;
+  if ((tile == T16x128 || tile == T128x16) && prec != VBG_PREC_FP32) return false;
   return true;
 }
 
-// Shapes a mixed launch holds (one kernel dispatching on each item's shape).
-bool mixable(int tile) { return tile >= T64 && tile <= T128; }
+// Shapes a mixed launch holds (one kernel dispatching on each item's shape):
+// 64x64 .. 128x128, and the skinny ones (FP32's mixed kernel holds them).
+bool mixable(int tile) { return (tile >= T64 && tile <= T128) || tile == T16x128 || tile == T128x16; }
 
 // AUTO: per-matrix minimum modelled cost, then, while the batch exposes fewer
 // work items than resident workgroup slots, split the matrix whose single
@@ -117,7 +121,7 @@ void choose_auto(const std::vector<Shape>& sh, int prec, const vbg_opts& o, int 
   const bool split_ok = prec != VBG_PREC_FP32_SEQ && o.split_k_min >= 0;
   int64_t items = 0;
   for (int b = 0; b < B; ++b) items += num_tiles(tile[b], sh[b]);
-  const int smaller[NTILES] = {-1, -1, -1, T64, T64, T128x64, T128};
+  const int smaller[NTILES] = {-1, -1, -1, T64, T64, T128x64, T128, T16, T16};
   for (int iter = 0; iter < 4 * B && items < slots; ++iter) {
     int pick = -1;
     double longest = 0;
@@ -358,6 +362,10 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
           err = "the 256x128 tile is built for bf16x3 only";
           return VBG_ERR_NOT_SUPPORTED;
         }
+        if ((o.fixed_tile == T16x128 || o.fixed_tile == T128x16) && prec != VBG_PREC_FP32) {
+          err = "the 16x128 and 128x16 tiles are built for FP32 only";
+          return VBG_ERR_NOT_SUPPORTED;
+        }
         std::fill(tile.begin(), tile.end(), o.fixed_tile);
         break;
       case VBG_POLICY_REFERENCE: {
@@ -574,10 +582,11 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
       const int r = M[b] % BMt, s = N[b] % BNt;
       const int t_bot = BNt == 128 ? T64x128 : T64;  // 64 x BN
       const int t_rgt = BMt == 128 ? T128x64 : T64;  // BM x 64
-      // (edge strips only for tiles with both sides <= 128)
-      const bool half_m = edges && BMt == 128 && BNt <= 128 && r > 0 && r <= 64 && tile_allowed(prec, t_bot, o.tile_mask) &&
+      // (edge strips only for tiles with both sides 64 or 128)
+      const bool sq = BMt >= 64 && BMt <= 128 && BNt >= 64 && BNt <= 128;
+      const bool half_m = edges && sq && BMt == 128 && r > 0 && r <= 64 && tile_allowed(prec, t_bot, o.tile_mask) &&
                           (BNt == 128 || tile_allowed(prec, T64, o.tile_mask));
-      const bool half_n = edges && BNt == 128 && BMt <= 128 && s > 0 && s <= 64 && tile_allowed(prec, t_rgt, o.tile_mask) &&
+      const bool half_n = edges && sq && BNt == 128 && s > 0 && s <= 64 && tile_allowed(prec, t_rgt, o.tile_mask) &&
                           (BMt == 128 || tile_allowed(prec, T64, o.tile_mask));
       const int rows = half_m ? M[b] / BMt : cdiv(M[b], BMt);
       const int cols = half_n ? N[b] / BNt : cdiv(N[b], BNt);
@@ -592,11 +601,12 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     const double kMixFrac = 0.12;
     build_parts(o.edge_tiles >= 0);  // the parts a mixed launch would have
     double cost[NTILES] = {};
-    for (int t = T64; t <= T128; ++t)
-      for (const Part& x : parts[t])
-        cost[t] += (double)(x.tm1 - x.tm0) * (x.tn1 - x.tn0) * item_cost(t, sh[x.b]);
+    for (int t = 0; t < NTILES; ++t)
+      if (mixable(t))
+        for (const Part& x : parts[t])
+          cost[t] += (double)(x.tm1 - x.tm0) * (x.tn1 - x.tn0) * item_cost(t, sh[x.b]);
     double all = 0.0, top = 0.0;
-    for (int t = T64; t <= T128; ++t) {
+    for (int t = 0; t < NTILES; ++t) {
       all += cost[t];
       top = std::max(top, cost[t]);
     }
@@ -627,8 +637,8 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   if (mix_family && !mixed && o.mixed_tiles == 0 && VBG_PARTIAL_MIX) {
     double best = 0.0;
     int others = 0;
-    for (int t = T64; t <= T128; ++t) {
-      if (parts[t].empty()) continue;
+    for (int t = 0; t < NTILES; ++t) {
+      if (parts[t].empty() || !mixable(t)) continue;
       double c = 0.0;
       for (const Part& x : parts[t]) c += (double)(x.tm1 - x.tm0) * (x.tn1 - x.tn0) * item_cost(t, sh[x.b]);
       if (keep < 0 || c > best) {

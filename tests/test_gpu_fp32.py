@@ -55,7 +55,8 @@ EDGE_SHAPES = ([1, 2, 15, 16, 17, 33, 63, 64, 65, 127, 129, 200, 31],
 
 
 @pytest.mark.parametrize("mixed", [1, -1])
-@pytest.mark.parametrize("tile", ["16x16", "32x32", "64x64", "128x64", "64x128", "128x128"])
+@pytest.mark.parametrize("tile", ["16x16", "32x32", "64x64", "128x64", "64x128", "128x128",
+                                  "16x128", "128x16"])
 @pytest.mark.parametrize("trans", ["NN", "NT", "TN", "TT"])
 def test_fp32_every_tile_every_transpose(bg, orc, gpu, tile, trans, mixed):
     """Every tile shape and transpose, in the mixed launch (items of several
@@ -129,7 +130,7 @@ def test_fp32_zero_k_and_empty(bg, orc, gpu):
     p.run(e, e, e, 1.0, 0.0)
 
 
-@pytest.mark.parametrize("tile", ["16x16", "32x32", "64x64", "128x128"])
+@pytest.mark.parametrize("tile", ["16x16", "32x32", "64x64", "128x128", "16x128", "128x16"])
 def test_fp32_split_k_forced(bg, orc, gpu, tile):
     """K cut into slices whose FP32 partial panels are summed by the split-K
     reduction (alpha and beta applied there)."""

@@ -103,7 +103,9 @@ def test_invalid_arguments_return_errors(bg):
     with pytest.raises(VbgError):
         bg.Plan([4], [4], [4], prec=9, host_only=True)
     with pytest.raises(VbgError):
-        bg.Plan([4], [4], [4], host_only=True, policy="fixed", fixed_tile=7)
+        bg.Plan([4], [4], [4], host_only=True, policy="fixed", fixed_tile=99)
+    with pytest.raises(VbgError):  # the skinny tiles are built for FP32 only
+        bg.Plan([4], [4], [4], host_only=True, policy="fixed", fixed_tile="16x128", prec="bf16x3")
     p = bg.Plan([4], [4], [4], host_only=True)
     L = bg.lib()
     rc = L.vbg_sgemm_vbatched(p._p, 1.0, None, None, 0.0, None, None)
