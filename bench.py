@@ -107,18 +107,26 @@ def per_matrix_library(vb, alpha, beta, steps, warmup):
     return time_launches(library_loop(vb, alpha, beta), steps, warmup)
 
 
-def cpu_baseline(M, N, K, alpha, beta, budget_s=12.0):
-    """Host per-matrix SGEMM loop (oracle/liboracle.so, OpenMP across matrices)
-    on a bounded sample of the same workload: matrices in random order (keeping
-    the 90/10 mix) until about `budget_s` seconds of CPU work."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from oracle import oracle as orc
-    rng = np.random.default_rng(123)
-    # the whole workload in random order, repeated (at most 4 passes) until
-    # about budget_s seconds of CPU work
-    order = np.concatenate([rng.permutation(len(M)) for _ in range(4)])
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_sample_rate(orc, M, N, K, alpha, beta, budget_s, nthreads, seed):
+    """Host per-matrix SGEMM (the oracle's blocked loop) over matrices of the
+    workload in random order (keeping its size mix), until about `budget_s`
+    seconds of CPU work: (GFLOP/s, matrices, GFLOP, seconds)."""
+    rng = np.random.default_rng(seed)
+    order = np.concatenate([rng.permutation(len(M)) for _ in range(4)])  # at most 4 passes
     done_flops, done_t, used = 0, 0.0, 0
-    chunk = 64
+    chunk = 64 if nthreads != 1 else 8
     while done_t < budget_s and used < len(order):
         idx = order[used: used + chunk]
         used += len(idx)
@@ -127,16 +135,29 @@ def cpu_baseline(M, N, K, alpha, beta, budget_s=12.0):
         Cs = [rng.uniform(-1, 1, int(M[i]) * int(N[i])).astype(np.float32) for i in idx]
         m, n, k = M[idx], N[idx], K[idx]
         t0 = time.perf_counter()
-        orc.sgemm_vbatched("N", "N", m, n, k, alpha, As, m, Bs, k, beta, Cs, m)
+        orc.sgemm_vbatched("N", "N", m, n, k, alpha, As, m, Bs, k, beta, Cs, m, nthreads=nthreads)
         done_t += time.perf_counter() - t0
         done_flops += int(2 * np.sum(m.astype(np.int64) * n * k))
+    return done_flops / done_t / 1e9, used, done_flops / 1e9, done_t
+
+
+def cpu_baseline(M, N, K, alpha, beta, budget_s=12.0):
+    """Host per-matrix SGEMM loop (oracle/liboracle.so, OpenMP across matrices)
+    on a bounded sample of the same workload: matrices in random order (keeping
+    the 90/10 mix) until about `budget_s` seconds of CPU work on every thread,
+    then a quarter of that on one thread."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as orc
+    rate, used, gf, secs = cpu_sample_rate(orc, M, N, K, alpha, beta, budget_s, 0, 123)
+    rate1, used1, gf1, secs1 = cpu_sample_rate(orc, M, N, K, alpha, beta, budget_s / 4, 1, 124)
     return {
-        "value": round(done_flops / done_t / 1e9, 2), "unit": "GFLOP/s",
-        "cores": orc.num_threads(), "kind": "port",
+        "value": round(rate, 2), "unit": "GFLOP/s",
+        "cores": orc.num_threads(), "kind": "port", "cpu_model": cpu_model(),
+        "single_thread_value": round(rate1, 2),
         "sample": f"{used} matrices ({used / len(M):.2f} passes over the {len(M)}) of the same "
-                  f"workload in random order "
-                  f"({done_flops / 1e9:.1f} GFLOP, {done_t:.1f} s): host per-matrix FP32 SGEMM, "
-                  f"OpenMP across matrices",
+                  f"workload in random order ({gf:.1f} GFLOP, {secs:.1f} s): host per-matrix "
+                  f"FP32 SGEMM (the oracle's cache-blocked loop), OpenMP across matrices; "
+                  f"single thread: {used1} matrices, {gf1:.1f} GFLOP, {secs1:.1f} s",
     }
 
 
