@@ -239,9 +239,9 @@ struct Bf16Tile {
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds_f) {
     char* lds = (char*)lds_f;
-    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
-    const u16* __restrict__ A = (const u16*)vbg_uptr(a.A[it.mat]);
-    const u16* __restrict__ B = (const u16*)vbg_uptr(a.B[it.mat]);
+    const vbg_mat_desc d = vbg_udesc(vbg_cload(a.desc, it.mat));
+    const u16* __restrict__ A = (const u16*)vbg_uptr(vbg_cload(a.A, it.mat));
+    const u16* __restrict__ B = (const u16*)vbg_uptr(vbg_cload(a.B, it.mat));
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;

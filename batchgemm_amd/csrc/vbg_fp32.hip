@@ -41,9 +41,9 @@ struct Fp32MfmaTile {
   using LB = TileLoader<BN, BK, P, !TB>;
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
-    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
-    const float* __restrict__ A = vbg_uptr(a.A[it.mat]);
-    const float* __restrict__ B = vbg_uptr(a.B[it.mat]);
+    const vbg_mat_desc d = vbg_udesc(vbg_cload(a.desc, it.mat));
+    const float* __restrict__ A = vbg_uptr(vbg_cload(a.A, it.mat));
+    const float* __restrict__ B = vbg_uptr(vbg_cload(a.B, it.mat));
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;
@@ -162,9 +162,9 @@ struct Fp32SmallTile {
   using LB = TileLoader<BMN, BK, P, !TB>;
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
-    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
-    const float* __restrict__ A = vbg_uptr(a.A[it.mat]);
-    const float* __restrict__ B = vbg_uptr(a.B[it.mat]);
+    const vbg_mat_desc d = vbg_udesc(vbg_cload(a.desc, it.mat));
+    const float* __restrict__ A = vbg_uptr(vbg_cload(a.A, it.mat));
+    const float* __restrict__ B = vbg_uptr(vbg_cload(a.B, it.mat));
     const int m0 = it.tm * BMN, n0 = it.tn * BMN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;
@@ -255,9 +255,9 @@ struct Fp32SkinnyTile {
   using LB = TileLoader<BN, BK, P, !TB>;
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
-    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
-    const float* __restrict__ A = vbg_uptr(a.A[it.mat]);
-    const float* __restrict__ B = vbg_uptr(a.B[it.mat]);
+    const vbg_mat_desc d = vbg_udesc(vbg_cload(a.desc, it.mat));
+    const float* __restrict__ A = vbg_uptr(vbg_cload(a.A, it.mat));
+    const float* __restrict__ B = vbg_uptr(vbg_cload(a.B, it.mat));
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;
@@ -344,10 +344,10 @@ struct SeqTile {
   }
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds) {
-    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
-    const float* A = vbg_uptr(a.A[it.mat]);
-    const float* B = vbg_uptr(a.B[it.mat]);
-    float* C = vbg_uptr(a.C[it.mat]);
+    const vbg_mat_desc d = vbg_udesc(vbg_cload(a.desc, it.mat));
+    const float* A = vbg_uptr(vbg_cload(a.A, it.mat));
+    const float* B = vbg_uptr(vbg_cload(a.B, it.mat));
+    float* C = vbg_uptr(vbg_cload(a.C, it.mat));
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     float* As = lds;
     float* Bs = lds + BK * PA;

@@ -257,18 +257,18 @@ struct SplitTile {
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, float* lds_f) {
     char* lds = (char*)lds_f;
-    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
-    const float* __restrict__ A = PK ? nullptr : vbg_uptr(a.A[it.mat]);
-    const float* __restrict__ B = PK ? nullptr : vbg_uptr(a.B[it.mat]);
+    const vbg_mat_desc d = vbg_udesc(vbg_cload(a.desc, it.mat));
+    const float* __restrict__ A = PK ? nullptr : vbg_uptr(vbg_cload(a.A, it.mat));
+    const float* __restrict__ B = PK ? nullptr : vbg_uptr(vbg_cload(a.B, it.mat));
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb, ke = kr.ke;
     const int nk = (ke - kb + BK - 1) / BK;
     float sA = 1.0f, sB = 1.0f, post = 1.0f;
     if (MODE != MODE_BF16X3) {
-      sA = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, a.scales[4 * it.mat])));
-      sB = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, a.scales[4 * it.mat + 1])));
-      post = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, a.scales[4 * it.mat + 2])));
+      sA = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, vbg_cload(a.scales, 4 * it.mat))));
+      sB = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, vbg_cload(a.scales, 4 * it.mat + 1))));
+      post = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, vbg_cload(a.scales, 4 * it.mat + 2))));
     }
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -290,8 +290,8 @@ struct SplitTile {
     LB lb;
     if constexpr (PK) {
       const int Kp = (d.K + VBG_PACK_KALIGN - 1) & ~(VBG_PACK_KALIGN - 1);
-      la.init(a.pk + a.pk_off[2 * it.mat], Kp, m0, d.M);
-      lb.init(a.pk + a.pk_off[2 * it.mat + 1], Kp, n0, d.N);
+      la.init(a.pk + vbg_cload(a.pk_off, 2 * it.mat), Kp, m0, d.M);
+      lb.init(a.pk + vbg_cload(a.pk_off, 2 * it.mat + 1), Kp, n0, d.N);
     } else {
       la.init(A, d.lda, m0, d.M, d.K);
       lb.init(B, d.ldb, n0, d.N, d.K);
@@ -587,14 +587,14 @@ struct DmaPkTile {
 
   __device__ static void run(const vbg_args& a, const vbg_item& it, char* s0, char* s1, char* s2, char* s3) {
     char* const st[4] = {s0, s1, s2, s3};
-    const vbg_mat_desc d = vbg_udesc(a.desc[it.mat]);
+    const vbg_mat_desc d = vbg_udesc(vbg_cload(a.desc, it.mat));
     const int m0 = it.tm * BM, n0 = it.tn * BN;
     const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
     const int kb = kr.kb;
     const int nk = (kr.ke - kb + BK - 1) / BK;
     float post = 1.0f;
     if (MODE != MODE_BF16X3)
-      post = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, a.scales[4 * it.mat + 2])));
+      post = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, vbg_cload(a.scales, 4 * it.mat + 2))));
     const int lane = threadIdx.x & 63, wave = vbg_u(threadIdx.x >> 6);
     const int wr = wave & 1, wc = wave >> 1;
     const int l32 = lane & 31, h = lane >> 5;
@@ -605,8 +605,8 @@ struct DmaPkTile {
     // read row MN-1 (their outputs are never stored).
     const int Kp = (d.K + VBG_PACK_KALIGN - 1) & ~(VBG_PACK_KALIGN - 1);
     const uint32_t planeA = (uint32_t)d.M * (uint32_t)Kp * EB, planeB = (uint32_t)d.N * (uint32_t)Kp * EB;
-    const v4u32_t rsA = vbg_rsrc_words(vbg_uptr(a.pk + vbg_u64(a.pk_off[2 * it.mat])), planeA * PLANES);
-    const v4u32_t rsB = vbg_rsrc_words(vbg_uptr(a.pk + vbg_u64(a.pk_off[2 * it.mat + 1])), planeB * PLANES);
+    const v4u32_t rsA = vbg_rsrc_words(vbg_uptr(a.pk + vbg_u64(vbg_cload(a.pk_off, 2 * it.mat))), planeA * PLANES);
+    const v4u32_t rsB = vbg_rsrc_words(vbg_uptr(a.pk + vbg_u64(vbg_cload(a.pk_off, 2 * it.mat + 1))), planeB * PLANES);
     int voff[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
