@@ -538,14 +538,14 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   std::vector<std::vector<Part>> parts(NTILES);
   // Mixed launch: the parts of every shape from 64x64 to 128x128 share one
   // persistent launch, so edge strips and medium matrices fill the big tiles'
-  // queues instead of running as launches of their own after them.  Measured
-  // on the unbalanced B=2048 batch (TFLOP/s, mixed+edges / separate+edges /
-  // separate without edges): FP32 96 / 109 / 106, bf16x3 156 / 209 / 218,
-  // FP16 206 / 192 / 202, FP8 206 / 192 / 202 -- the combined kernel loses
-  // the 128x128 body's accumulator-in-AGPR allocation that FP32 and bf16x3
-  // run fastest with.  Default (0): mixed for FP16 / FP8 and for small
-  // batches; edge tiles where the launches are mixed and for the FP32 and
-  // BF16 families.
+  // queues instead of running as launches of their own after them.  FP32 runs
+  // mixed always: its mixed kernel runs 128x128 items as fast as the one-shape
+  // kernel (256 x 512x512x2048: 134 TFLOP/s either way), so one launch saves
+  // the second launch's start and tail (unbalanced B=2048 batch: 7.30-7.32 ms
+  // against 7.36-7.38 with the edges and medium matrices in a launch of their
+  // own after the 128x128 one).  Default (0): mixed for FP32, FP16 and FP8 and
+  // for small batches; edge tiles where the launches are mixed and for the
+  // FP32 and BF16 families.
   //   A batch whose launches would all be short (modelled total under
   // kSmallBatch units: every launch is start-up latency) also runs mixed: one
   // launch instead of one per shape.
@@ -567,9 +567,9 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   //   Otherwise the plan runs mixed when more than kMixFrac of the modelled
   // cost of the 64x64..128x128 parts lies outside the largest shape: those
   // launches would run serially after the dominant one.  Measured on the
-  // batching workload (log-uniform sizes, about 25% outside): mixed is
-  // +11% for FP32 and +13% for bf16x3; on the unbalanced B=2048 batch (about
-  // 4% outside) separate launches stay faster.
+  // batching workload (log-uniform sizes, about 25% outside): mixed is +13%
+  // for bf16x3; on the unbalanced B=2048 batch (about 4% outside) separate
+  // launches stay faster for bf16x3.
   const bool mix_family = fam == FAM_FP32 || fam == FAM_SPLIT || fam == FAM_SPLIT_PK;
   auto build_parts = [&](bool edges) {
     for (auto& v : parts) v.clear();
@@ -596,7 +596,8 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
       if (half_m && half_n) parts[T64].push_back({b, 2 * rows, 2 * rows + 1, 2 * cols, 2 * cols + 1});
     }
   };
-  bool mixed = mix_family && (o.mixed_tiles > 0 || (o.mixed_tiles == 0 && (split_scaled || small_batch)));
+  bool mixed = mix_family && (o.mixed_tiles > 0 ||
+                              (o.mixed_tiles == 0 && (fam == FAM_FP32 || split_scaled || small_batch)));
   if (mix_family && o.mixed_tiles == 0 && !mixed) {
     const double kMixFrac = 0.12;
     build_parts(o.edge_tiles >= 0);  // the parts a mixed launch would have

@@ -127,8 +127,10 @@ def unbalanced(bg, gpu):
 def test_unbalanced_batch_fp32_and_bf16x3_vs_fp64(unbalanced, prec):
     w = unbalanced
     plan = w.run(prec)
-    if prec == "fp32":  # the benchmark's plan: the 128x128 launch and the mixed one
-        assert plan.n_launches >= 2 and plan.n_items > 20000
+    if prec == "fp32":  # the benchmark's plan: every tile shape in one mixed launch
+        assert plan.n_launches == 1 and plan.n_items > 20000
+        from batchgemm_amd import _lib
+        assert int(plan.kernels()[0, 1]) == _lib.TILE_MIXED
     num = den = 0.0
     worst = (0.0, None, 0.0)
     for b in w.sel:
