@@ -70,16 +70,16 @@ bool is_split(int prec) {
 }
 
 // The split-precision and BF16 families are built on 32x32 MFMA blocks only
-// (>= 64x64); the 256x128 tile for bf16x3 only, and AUTO uses it only when
-// the tile mask names it: with it, the unbalanced B=2048 bf16x3 batch ran
-// 3.700 vs 3.714 ms (the 21 large matrices it leaves on 128x128 tiles run as
-// a second launch after it), and batches of the batching workload 4-11%
-// slower (two launches in series; one combined kernel spilled registers and
-// was no faster).  On one shape alone it gains 5-10% (SplitTile notes).
+// (>= 64x64); the 256x128 tile for bf16x3 only.  AUTO picks it per matrix by
+// the cost model like any other shape (unbalanced B=2048 bf16x3 batch 3.567
+// vs 3.644-3.684 ms on 128x128 alone, the matrices it leaves to smaller
+// shapes running as a second, short launch; batching workload 97.7 vs 95.9
+// TFLOP/s).  A combined kernel holding the 256x128 body with the others
+// spills registers, so the shape keeps a launch of its own.
 bool tile_allowed(int prec, int tile, unsigned mask) {
   if (mask && !(mask & (1u << tile))) return false;
   if (is_split(prec) && tile < T64) return false;
-  if (tile == T256x128 && (prec != VBG_PREC_BF16X3 || !(mask & (1u << T256x128)))) return false;
+  if (tile == T256x128 && prec != VBG_PREC_BF16X3) return false;
   if ((tile == T16x128 || tile == T128x16) && prec != VBG_PREC_FP32) return false;
   return true;
 }

@@ -87,8 +87,8 @@ typedef struct vbg_opts {
   int wavefront_thres;  /* REFERENCE: cascade threshold (reference default 6000) */
   int tlp_thres;        /* WANG / LI: TLP threshold */
   int fixed_tile;       /* FIXED: vbg_tile id */
-  unsigned tile_mask;   /* AUTO: bit t enables tile t (0 = all but 256x128, which AUTO uses
-                           only when its bit is set) */
+  unsigned tile_mask;   /* AUTO: bit t enables tile t (0 = all; each precision uses the
+                           shapes built for it) */
   int big_tile_min_mn;  /* AUTO: min(M,N) at which 128x128 becomes eligible (0 = default) */
   int split_k_min;      /* split K of matrices with K >= this whose one item would be the
                            launch's critical path (0 = 256, <0 = never split) */

@@ -335,17 +335,16 @@ def test_package_dataset_is_the_reference_dataset(bg):
     assert open(path, "rb").read() == open(DATA, "rb").read()
 
 
-def test_256x128_tile_is_opt_in_and_bf16x3_only(bg):
-    """AUTO plans use the 256x128 tile only when the tile mask names it, and
-    only for bf16x3; FIXED refuses it for other precisions."""
+def test_256x128_tile_is_bf16x3_only(bg):
+    """AUTO plans use the 256x128 tile for bf16x3 only (unless the tile mask
+    leaves it out); FIXED refuses it for other precisions."""
     from batchgemm_amd import workloads as w
     M, N, K = w.shapes_unbalanced(256, seed=4)
     all_but = sum(1 << t for t in range(6))
-    with_256 = all_but | (1 << bg.TILES["256x128"])
     for prec in ("fp32", "bf16x3", "fp16", "fp8"):
-        assert not (bg.Plan(M, N, K, prec=prec, host_only=True).tiles() == 6).any()
-        uses = (bg.Plan(M, N, K, prec=prec, host_only=True, tile_mask=with_256).tiles() == 6).any()
+        uses = (bg.Plan(M, N, K, prec=prec, host_only=True).tiles() == 6).any()
         assert uses == (prec == "bf16x3"), prec
+        assert not (bg.Plan(M, N, K, prec=prec, host_only=True, tile_mask=all_but).tiles() == 6).any()
     p = bg.Plan(M, N, K, prec="bf16x3", host_only=True, policy="fixed", fixed_tile="256x128")
     assert (p.tiles()[(M > 0) & (N > 0)] == 6).all()
     _cover(p, M, N)
