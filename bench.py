@@ -81,22 +81,52 @@ def kernel_name(bg, prec, kind, tile):
     return f"{KERNEL_TEMPLATE[kind]}<{bm},{bn}>" + tag
 
 
+TRAFFIC_FILE = os.path.join("profiles", "traffic_unbalanced_b2048.txt")
+
+
+def source_digest():
+    """sha256 over the library's sources (batchgemm_amd/csrc/*, its Makefile
+    and include/vbgemm.h, in name order): names the code a profile was
+    measured on, so a figure measured on other code is not reported as this
+    code's."""
+    import hashlib
+    root = os.path.dirname(os.path.abspath(__file__))
+    csrc = os.path.join(root, "batchgemm_amd", "csrc")
+    files = [os.path.join(csrc, n) for n in sorted(os.listdir(csrc))
+             if n.endswith((".hip", ".cpp", ".h"))]
+    files += [os.path.join(root, "batchgemm_amd", "Makefile"), os.path.join(root, "include", "vbgemm.h")]
+    h = hashlib.sha256()
+    for path in files:
+        h.update(os.path.basename(path).encode() + b"\0")
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
-    (FETCH_SIZE, WRITE_SIZE, calibrated on a copy kernel) and kept in
-    profiles/traffic_unbalanced_b2048.txt as lines `<kernel> <bytes>`;
-    None when that kernel was not measured."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "traffic_unbalanced_b2048.txt")
+    (FETCH_SIZE, WRITE_SIZE, calibrated on a copy kernel), from the lines
+    `<kernel> <bytes>` of TRAFFIC_FILE, whose header names the commit and the
+    source digest it was measured on.  Returns (bytes or None, commit or
+    None): bytes only when that kernel was measured on these sources."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    digest = commit = None
+    found = None
     try:
         with open(path) as f:
             for ln in f:
                 parts = ln.split()
-                if len(parts) == 2 and parts[0] == kernel and not ln.startswith("#"):
-                    return int(float(parts[1]))
+                if ln.startswith("# source-digest:") and len(parts) >= 3:
+                    digest = parts[2]
+                elif ln.startswith("# commit:") and len(parts) >= 3:
+                    commit = parts[2]
+                elif len(parts) == 2 and parts[0] == kernel and not ln.startswith("#"):
+                    found = int(float(parts[1]))
     except OSError:
-        pass
-    return None
+        return None, None
+    if found is None or digest != source_digest():
+        return None, commit
+    return found, commit
 
 
 def per_matrix_library(vb, alpha, beta, steps, warmup):
@@ -286,9 +316,10 @@ def main():
                     help="for rocprofv3 --pmc runs: one 1 GiB device copy (the calibration "
                          "kernel of known traffic), then --steps runs of the workload's plan "
                          "for each --prec; no timing, no extras")
-    ap.add_argument("--workload", default="unbalanced", choices=["unbalanced", "uniform"],
-                    help="--traffic-probe only: the unbalanced B=2048 batch (default) or a "
-                         "uniform 64 x 2048^3 batch on 128x128 tiles")
+    ap.add_argument("--workload", default="unbalanced", choices=["unbalanced", "uniform", "batching"],
+                    help="--traffic-probe only: the unbalanced B=2048 batch (default), a "
+                         "uniform 64 x 2048^3 batch on 128x128 tiles, or the first batch of the "
+                         "batching workload (B=7007, M,N,K ~ log-U[8,1024], alpha 1, beta 0)")
     args = ap.parse_args()
     precs = args.prec.split(",")
     for pr in precs:
@@ -310,6 +341,10 @@ def main():
     alpha, beta = 1.5, 0.5
     if args.traffic_probe and args.workload == "uniform":
         M = N = K = np.full(64, 2048, np.int32)
+    elif args.traffic_probe and args.workload == "batching":
+        from batchgemm_amd.autotune import batching_workload
+        M, N, K = (np.asarray(x, np.int32) for x in batching_workload(50, args.seed)[0])
+        alpha, beta = 1.0, 0.0
     else:
         M, N, K = w.shapes_unbalanced(2048, seed=args.seed)
     flops = w.flops(M, N, K)
@@ -352,7 +387,7 @@ def main():
     log(f"[bench] kernels: " + "; ".join(f"{r['kernel']} {r['ms']} ms {r['gflop']} GFLOP" for r in rows))
     log(f"[bench] dominant kernel {rows[dom]['kernel']}: {dom_ms:.3f} ms, "
         f"{achieved / 1e12:.1f} TFLOP/s")
-    traffic = read_traffic(rows[dom]["kernel"])
+    traffic, traffic_commit = read_traffic(rows[dom]["kernel"])
 
     line = {
         "metric": "batched GFLOP/s (2*sum(MNK)/t), unbalanced variable-size FP32 batched GEMM",
@@ -375,6 +410,9 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 2),
                      "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "traffic_measured_on": {"commit": traffic_commit, "file": TRAFFIC_FILE,
+                                             "source_digest": source_digest(),
+                                             "current": traffic is not None},
                      "kernel": rows[dom]["kernel"], "launch_ms": round(dom_ms, 4),
                      "launch_gflop": rows[dom]["gflop"], "kernels": rows},
         "cpu_baseline": None,
