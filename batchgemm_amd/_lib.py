@@ -27,10 +27,10 @@ VBG_ERR_UNDEFINED = -6
 PREC = {"fp32": 0, "bf16x3": 1, "fp16": 2, "fp8": 3, "fp32_seq": 4, "bf16": 5, "bf16_out_bf16": 6}
 POLICY = {"auto": 0, "reference": 1, "fixed": 2, "wang": 3, "li": 4}
 TILES = {"16x16": 0, "32x32": 1, "64x64": 2, "128x64": 3, "64x128": 4, "128x128": 5, "256x128": 6,
-         "16x128": 7, "128x16": 8}
+         "16x128": 7, "128x16": 8, "256x256": 9}
 TILE_SHAPES = [(16, 16), (32, 32), (64, 64), (128, 64), (64, 128), (128, 128), (256, 128), (16, 128),
-               (128, 16)]
-TILE_MIXED = 9  # launch tile id of a mixed launch (query 7): the shape is per item
+               (128, 16), (256, 256)]
+TILE_MIXED = 10  # launch tile id of a mixed launch (query 7): the shape is per item
 
 # Every symbol include/vbgemm.h declares (checked by the CPU test-suite).
 EXPORTS = [

@@ -61,18 +61,18 @@ struct vbg_args {
 };
 
 // Tile ids (mirror vbg_tile in include/vbgemm.h).  256x128 is built for the
-// bf16x3 family only.
-// 16x128 and 128x16 (skinny: one side of at most 16) for the FP32 family only.
+// bf16x3 and BF16 families, 256x256 for the BF16 family only, 16x128 and
+// 128x16 (skinny: one side of at most 16) for the FP32 family only.
 enum {
   T16 = 0, T32 = 1, T64 = 2, T128x64 = 3, T64x128 = 4, T128 = 5, T256x128 = 6, T16x128 = 7, T128x16 = 8,
-  NTILES = 9
+  T256 = 9, NTILES = 10
 };
 // Launch tile id of a mixed launch: one persistent kernel whose items are of
 // the shapes 64x64, 128x64, 64x128 and 128x128 (and, FP32, 16x128 and
 // 128x16), each carrying its own tile id.
-enum { T_MIXED = 9 };
-static const int kTileBM[NTILES] = {16, 32, 64, 128, 64, 128, 256, 16, 128};
-static const int kTileBN[NTILES] = {16, 32, 64, 64, 128, 128, 128, 128, 16};
+enum { T_MIXED = 10 };
+static const int kTileBM[NTILES] = {16, 32, 64, 128, 64, 128, 256, 16, 128, 256};
+static const int kTileBN[NTILES] = {16, 32, 64, 64, 128, 128, 128, 128, 16, 256};
 
 // Work-queue count of the XCD-aware scheduler: MI355X has 8 XCDs and
 // dispatches consecutive workgroup ids round-robin across them.

@@ -31,7 +31,7 @@ namespace vbg {
 namespace {
 
 // k-tile depth of each tile id (matches the kernels).
-const int kTileBK[NTILES] = {64, 32, 32, 32, 32, 32, 16, 64, 64};
+const int kTileBK[NTILES] = {64, 32, 32, 32, 32, 32, 16, 64, 64, 64};
 
 // Relative MFMA efficiency of each tile shape on gfx950 (per-flop cost factor
 // = 1/eff).  Larger tiles reuse each LDS byte and each HBM byte more times.
@@ -39,8 +39,11 @@ const int kTileBK[NTILES] = {64, 32, 32, 32, 32, 32, 16, 64, 64};
 // TFLOP/s against 128x128 (two k-tiles of 16 against one of 32 per barrier,
 // 25% fewer operand bytes per flop).  The skinny 16x128 / 128x16 tiles
 // (FP32 only): 0.35, between the 16x16 and 32x32 tiles -- they stream the
-// long operand at an eighth of a 128x128 tile's MFMA work per byte.
-const double kTileEff[NTILES] = {0.20, 0.40, 0.70, 0.85, 0.85, 1.00, 1.10, 0.35, 0.35};
+// long operand at an eighth of a 128x128 tile's MFMA work per byte.  256x256
+// (BF16 only): 1.40 -- half the operand bytes per flop of 128x128, the bytes
+// the LDS-DMA address path can deliver per MFMA cycle being what bounds the
+// BF16 tiles.
+const double kTileEff[NTILES] = {0.20, 0.40, 0.70, 0.85, 0.85, 1.00, 1.10, 0.35, 0.35, 1.40};
 // Fixed cost of one work item in units of a 128x128x32 k-step (queue pop,
 // pipeline fill, epilogue).
 const double kItemFixed = 3.0;
@@ -79,7 +82,9 @@ bool is_split(int prec) {
 bool tile_allowed(int prec, int tile, unsigned mask) {
   if (mask && !(mask & (1u << tile))) return false;
   if (is_split(prec) && tile < T64) return false;
-  if (tile == T256x128 && prec != VBG_PREC_BF16X3) return false;
+  const bool bf16 = prec == VBG_PREC_BF16 || prec == VBG_PREC_BF16_OUT_BF16;
+  if (tile == T256x128 && prec != VBG_PREC_BF16X3 && !bf16) return false;
+  if (tile == T256 && !bf16) return false;
   if ((tile == T16x128 || tile == T128x16) && prec != VBG_PREC_FP32) return false;
   return true;
 }
@@ -121,7 +126,7 @@ void choose_auto(const std::vector<Shape>& sh, int prec, const vbg_opts& o, int 
   const bool split_ok = prec != VBG_PREC_FP32_SEQ && o.split_k_min >= 0;
   int64_t items = 0;
   for (int b = 0; b < B; ++b) items += num_tiles(tile[b], sh[b]);
-  const int smaller[NTILES] = {-1, -1, -1, T64, T64, T128x64, T128, T16, T16};
+  const int smaller[NTILES] = {-1, -1, -1, T64, T64, T128x64, T128, T16, T16, T256x128};
   for (int iter = 0; iter < 4 * B && items < slots; ++iter) {
     int pick = -1;
     double longest = 0;
@@ -358,8 +363,8 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
           err = "fixed_tile out of range";
           return VBG_ERR_INVALID_ARG;
         }
-        if (o.fixed_tile == T256x128 && prec != VBG_PREC_BF16X3) {
-          err = "the 256x128 tile is built for bf16x3 only";
+        if ((o.fixed_tile == T256x128 || o.fixed_tile == T256) && !tile_allowed(prec, o.fixed_tile, 0)) {
+          err = "the 256x128 tile is built for bf16x3 and BF16 operands, 256x256 for BF16 only";
           return VBG_ERR_NOT_SUPPORTED;
         }
         if ((o.fixed_tile == T16x128 || o.fixed_tile == T128x16) && prec != VBG_PREC_FP32) {
@@ -683,7 +688,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     // Measured on the grouped BF16 passes, whose big group is one such part:
     // forward 0.479 vs 0.473-0.480 ms, backward 1.007 vs 1.014 ms -- no gain,
     // so L2 locality across XCDs does not bound those tiles.
-    if (hp.nq > 1 && VBG_BAND_SPLIT) {
+    if (hp.nq > 1 && (VBG_BAND_SPLIT || o.xcd_queues >= 2)) {
       double total = 0.0;
       for (const TPart& x : ps) total += x.cost;
       const double share = total / hp.nq;

@@ -515,40 +515,6 @@ vbg_split_kernel(vbg_args a) {
 #define VBG_PK_DMA 1
 #endif
 
-// One LDS-DMA instruction: 16 bytes per lane from the buffer `rs` at byte
-// offset voff + soff into LDS at lds_base + 16 * lane.  Issued as inline asm
-// (it has no VGPR result: its destination is LDS) so that the compiler's
-// load-counter model does not see it and inserts no waits of its own for it
-// (merging its per-buffer DMA tracking at the loop back-edge, it drained every
-// DMA in flight at the top of each unrolled group of k-tiles).  The waits are
-// the explicit vmcnt(N) before each k-tile's barrier; no compiler-visible
-// vector-memory load is issued between the first DMA and the final vmcnt(0),
-// so those counts are exact.  One wait state separates the M0 write from the
-// DMA that reads it.
-__device__ __forceinline__ void vbg_dma16(v4u32_t rs, uint32_t lds_base, int voff, int soff) {
-  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, %3 offen lds"
-               ::"s"(lds_base), "v"(voff), "s"(rs), "s"(soff)
-               : "memory", "m0");
-}
-
-// Buffer descriptor words (base, stride 0, `bytes` records, raw dword
-// format) in scalar registers, for vbg_dma16.
-__device__ __forceinline__ v4u32_t vbg_rsrc_words(const void* p, uint32_t bytes) {
-  const uint64_t a = (uint64_t)p;
-  return v4u32_t{(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a),
-                 (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32) & 0xffffu),
-                 (uint32_t)__builtin_amdgcn_readfirstlane(bytes), 0x00020000u};
-}
-
-// s_waitcnt vmcnt(N) lgkmcnt(0), then s_barrier, as one compiler memory
-// barrier (no LDS access moves across it); earlier DMAs beyond the newest N
-// stay in flight (__syncthreads() would wait for all of them).
-template <int N>
-__device__ __forceinline__ void vbg_wait_vm_barrier() {
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
-}
-
 template <int MODE>
 struct DmaBK;  // k-tile depth: 64-byte LDS rows
 template <>

@@ -87,6 +87,51 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t vbg_rsrc(const void* p, uint32
 }
 
 // ---------------------------------------------------------------------------
+// LDS-DMA: buffer loads whose destination is LDS (buffer_load_dwordx4 ... lds).
+// One LDS-DMA instruction: 16 bytes per lane from the buffer `rs` at byte
+// offset voff + soff into LDS at lds_base + 16 * lane.  Issued as inline asm
+// (it has no VGPR result: its destination is LDS) so that the compiler's
+// load-counter model does not see it and inserts no waits of its own for it
+// (merging its per-buffer DMA tracking at the loop back-edge, it drained every
+// DMA in flight at the top of each unrolled group of k-tiles).  The waits are
+// the explicit vmcnt(N) before each k-tile's barrier; no compiler-visible
+// vector-memory load is issued between the first DMA and the final vmcnt(0),
+// so those counts are exact.  One wait state separates the M0 write from the
+// DMA that reads it.
+__device__ __forceinline__ void vbg_dma16(v4u32_t rs, uint32_t lds_base, int voff, int soff) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, %3 offen lds"
+               ::"s"(lds_base), "v"(voff), "s"(rs), "s"(soff)
+               : "memory", "m0");
+}
+
+// Buffer descriptor words (base, stride 0, `bytes` records, raw dword
+// format) in scalar registers, for vbg_dma16.
+__device__ __forceinline__ v4u32_t vbg_rsrc_words(const void* p, uint32_t bytes) {
+  const uint64_t a = (uint64_t)p;
+  return v4u32_t{(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a),
+                 (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32) & 0xffffu),
+                 (uint32_t)__builtin_amdgcn_readfirstlane(bytes), 0x00020000u};
+}
+
+// s_waitcnt vmcnt(N) lgkmcnt(0), then s_barrier, with no memory access moved
+// across either; earlier DMAs beyond the newest N stay in flight
+// (__syncthreads() would wait for all of them).  The wait is the builtin, not
+// inline asm, so the compiler's own wait tracking learns that every LDS read
+// issued before it has completed (after an opaque asm it would instead wait
+// for all LDS reads at the next use of any of them).
+template <int N>
+__device__ __forceinline__ void vbg_wait_vm_barrier() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  // gfx9 s_waitcnt immediate: vmcnt[3:0] at 3:0 and vmcnt[5:4] at 15:14,
+  // expcnt at 6:4 (7: no wait), lgkmcnt at 11:8
+  constexpr int imm = (N & 15) | ((N >> 4) << 14) | (7 << 4);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(imm);
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// ---------------------------------------------------------------------------
 // Operand tile loader.  Stages a ROWS x BK slice of one operand -- rows are the
 // operand's M (for A) or N (for B) index "mn", columns the reduction index k --
 // through registers.  r[v] holds 4 consecutive k of one row (coords(v) names

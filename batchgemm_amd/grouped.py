@@ -125,11 +125,13 @@ def grouped_gemm(A: torch.Tensor, B: torch.Tensor, m_sizes, out_dtype=torch.bflo
     return GroupedGemmFunction.apply(A, B, plan, out_dtype, grad_dtype)
 
 
-def bench_config4(steps: int = 20, warmup: int = 3, seed: int = 0) -> dict:
+def bench_config4(steps: int = 20, warmup: int = 3, seed: int = 0, library_reference: bool = False) -> dict:
     """Grouped forward and backward on the differentiation workload: G = 128
     groups, K = 2048, N = 1408, Zipf(1.2) row counts summing to 32768, BF16
     operands ~ N(0, 0.02), BF16 dC ~ N(0, 1), BF16 outputs; mean ms per call
-    over `steps` calls after `warmup`, from HIP events on the current stream."""
+    over `steps` calls after `warmup`, from HIP events on the current stream.
+    With `library_reference`, also the largest group's products through
+    torch.mm (hipBLASLt) on the same device."""
     from .workloads import zipf_groups
     m_sizes = zipf_groups(128, 32768, 1.2, seed=seed)
     K, N, S = 2048, 1408, 32768
@@ -164,9 +166,27 @@ def bench_config4(steps: int = 20, warmup: int = 3, seed: int = 0) -> dict:
 
     f_ms, b_ms = timed(fwd), timed(bwd)
     gfl = 2.0 * S * K * N / 1e9
-    return {"workload": "G=128, K=2048, N=1408, Zipf(1.2) rows summing to 32768, BF16 in/out, FP32 accumulate",
-            "fwd_ms": round(f_ms, 4), "bwd_ms": round(b_ms, 4), "fwd_bwd_ms": round(f_ms + b_ms, 4),
-            "fwd_tflops": round(gfl / f_ms, 1), "bwd_tflops": round(2 * gfl / b_ms, 1)}
+    out = {"workload": "G=128, K=2048, N=1408, Zipf(1.2) rows summing to 32768, BF16 in/out, FP32 accumulate",
+           "fwd_ms": round(f_ms, 4), "bwd_ms": round(b_ms, 4), "fwd_bwd_ms": round(f_ms + b_ms, 4),
+           "fwd_tflops": round(gfl / f_ms, 1), "bwd_tflops": round(2 * gfl / b_ms, 1)}
+    if library_reference:
+        # The largest group's three products as single 2-D torch.mm calls
+        # (hipBLASLt BF16 GEMMs, FP32 accumulate, BF16 out) on the same
+        # tensors: a same-device reference point for the dense part of each
+        # pass (the small groups are not included).
+        g0 = int(np.argmax(m_sizes))
+        r0 = int(np.sum(m_sizes[:g0]))
+        Ag, dCg, Bg = A[r0:r0 + m_sizes[g0]], dC[r0:r0 + m_sizes[g0]], B[g0]
+        lib_f = timed(lambda: torch.mm(Ag, Bg))
+        lib_da = timed(lambda: torch.mm(dCg, Bg.t()))
+        lib_db = timed(lambda: torch.mm(Ag.t(), dCg))
+        gfl0 = 2.0 * int(m_sizes[g0]) * K * N / 1e9
+        out["library_largest_group"] = {
+            "what": f"torch.mm (hipBLASLt) on the largest group, {int(m_sizes[g0])} x {K} x {N} BF16",
+            "fwd_ms": round(lib_f, 4), "da_ms": round(lib_da, 4), "db_ms": round(lib_db, 4),
+            "fwd_tflops": round(gfl0 / lib_f, 1), "da_tflops": round(gfl0 / lib_da, 1),
+            "db_tflops": round(gfl0 / lib_db, 1)}
+    return out
 
 
 if __name__ == "__main__":
@@ -176,7 +196,10 @@ if __name__ == "__main__":
     ap.add_argument("--bench", action="store_true", help="time the G=128, K=2048, N=1408 workload")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--library-reference", action="store_true",
+                    help="also time the largest group's products with torch.mm (hipBLASLt)")
     args = ap.parse_args()
     if not args.bench:
         ap.error("nothing to do: pass --bench")
-    print(json.dumps(bench_config4(args.steps, args.warmup)), flush=True)
+    print(json.dumps(bench_config4(args.steps, args.warmup, library_reference=args.library_reference)),
+          flush=True)

@@ -76,10 +76,11 @@ typedef enum vbg_tile {
   VBG_TILE_128x64 = 3,
   VBG_TILE_64x128 = 4,
   VBG_TILE_128x128 = 5,
-  VBG_TILE_256x128 = 6,     /* bf16x3 only: 4 waves of 128 x 64 each, 16-deep k-tiles */
+  VBG_TILE_256x128 = 6,     /* bf16x3 and BF16 operands only: 4 waves of 128 x 64 each */
   VBG_TILE_16x128 = 7,      /* FP32 only, M <= 16: one item spans 128 columns */
   VBG_TILE_128x16 = 8,      /* FP32 only, N <= 16: one item spans 128 rows */
-  VBG_TILE_COUNT = 9
+  VBG_TILE_256x256 = 9,     /* BF16 operands only: 4 waves of 128 x 128, 64-deep k-tiles */
+  VBG_TILE_COUNT = 10
 } vbg_tile;
 
 typedef struct vbg_opts {
@@ -93,7 +94,9 @@ typedef struct vbg_opts {
   int split_k_min;      /* split K of matrices with K >= this whose one item would be the
                            launch's critical path (0 = 256, <0 = never split) */
   int grid_wgs;         /* persistent grid size per kernel (0 = CUs x occupancy) */
-  int xcd_queues;       /* 1 = XCD-aware per-XCD work queues with stealing, 0 = one queue */
+  int xcd_queues;       /* 1 = XCD-aware per-XCD work queues with stealing, 0 = one queue;
+                           2 = per-XCD queues, and a matrix with more work than one queue's
+                           share is first cut into bands of tile rows, one per share */
   int host_only;        /* 1 = plan on the host only (no device upload, cannot run);
                            for inspection and tests on machines without a GPU */
   int num_cus;          /* host_only: CU count to plan for (0 = 256) */
@@ -204,7 +207,7 @@ int vbg_sgemm_vbatched_noalloc(int batch, const int* M, const int* N, const int*
  *   6 work items as (mat, tile id, tile_m, tile_n, k slice) x n (out[5*n]); tile_m / tile_n
  *     count in units of that item's tile shape
  *   7 kernel table, one row of (kind, tile id, first item, item count, grid) per kernel a run
- *     issues, in issue order (out[5*rows]); tile id 9 = a mixed launch (items of the 64x64 ..
+ *     issues, in issue order (out[5*rows]); tile id 10 = a mixed launch (items of the 64x64 ..
  *     128x128 shapes and, FP32, the skinny ones, see query 6); kind 0 = scale pre-pass (FP16 / FP8), 1 = FP32
  *     MFMA tiles, 2 = FP32 sequential-fma tiles, 3 = split-precision tiles, 4 = BF16 tiles,
  *     5 = split-K reduction, 6 = operand pack pre-pass (grid = pack jobs), 7 = split-precision

@@ -4,7 +4,8 @@ shapes (needs an MI355X).
 M, N and K that are not multiples of 4 or 8, every transpose pair, and leading
 dimensions padded past the row count (multiples of 8, as the family
 requires): the loaders' whole-vector and straddling paths, for operands stored
-k-contiguous and mn-contiguous.  The reference is FP64 on the same BF16
+k-contiguous and mn-contiguous, on the default tiles and on the 256x128
+and 256x256 tiles.  The reference is FP64 on the same BF16
 values; the kernels form exact BF16 products and accumulate in FP32, so the
 error is bounded by about K * 2^-24 of sum |a||b| (checked at 1e-5 of that
 sum, plus the final FP32 or BF16 rounding of C)."""
@@ -30,12 +31,13 @@ def up8(x):
     return (x + 7) // 8 * 8
 
 
+@pytest.mark.parametrize("tile", ["auto", "256x128", "256x256"])
 @pytest.mark.parametrize("trans", ["NN", "NT", "TN", "TT"])
 @pytest.mark.parametrize("prec", ["bf16", "bf16_out_bf16"])
-def test_bf16_plan_ragged_vs_fp64(bg, gpu, trans, prec):
+def test_bf16_plan_ragged_vs_fp64(bg, gpu, trans, prec, tile):
     import torch
     from batchgemm_amd.api import pointer_array
-    say(f"[bf16 plan] {prec} trans={trans}")
+    say(f"[bf16 plan] {prec} trans={trans} tile={tile}")
     ta, tb = trans[0] == "T", trans[1] == "T"
     g = torch.Generator(device="cuda").manual_seed(7)
     out_dt = torch.bfloat16 if prec == "bf16_out_bf16" else torch.float32
@@ -49,7 +51,8 @@ def test_bf16_plan_ragged_vs_fp64(bg, gpu, trans, prec):
         Cs.append(torch.randn(n, m, generator=g, device="cuda").to(out_dt))
         lda.append(la); ldb.append(lb); ldc.append(m)
     C0 = [c.clone() for c in Cs]
-    plan = bg.Plan(M, N, K, lda, ldb, ldc, trans[0], trans[1], prec=prec)
+    opts = {} if tile == "auto" else {"policy": "fixed", "fixed_tile": tile}
+    plan = bg.Plan(M, N, K, lda, ldb, ldc, trans[0], trans[1], prec=prec, **opts)
     alpha, beta = 1.5, 0.5
     plan.run(pointer_array(As), pointer_array(Bs), pointer_array(Cs), alpha, beta)
     torch.cuda.synchronize()

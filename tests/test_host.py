@@ -335,9 +335,10 @@ def test_package_dataset_is_the_reference_dataset(bg):
     assert open(path, "rb").read() == open(DATA, "rb").read()
 
 
-def test_256x128_tile_is_bf16x3_only(bg):
-    """AUTO plans use the 256x128 tile for bf16x3 only (unless the tile mask
-    leaves it out); FIXED refuses it for other precisions."""
+def test_256x128_tile_is_bf16x3_and_bf16_only(bg):
+    """AUTO plans use the 256x128 tile for bf16x3 and BF16 operands only
+    (unless the tile mask leaves it out); FIXED refuses it for other
+    precisions."""
     from batchgemm_amd import workloads as w
     M, N, K = w.shapes_unbalanced(256, seed=4)
     all_but = sum(1 << t for t in range(6))
