@@ -402,6 +402,10 @@ struct Bf16Tile {
 #ifndef VBG_BF16_DMA
 #define VBG_BF16_DMA 1
 #endif
+// k depth of the 256x256 tile's stages: 64 (a ring of 2 x 64 KB) or 32 (4 x 32 KB)
+#ifndef VBG_BF16_HUGE_BK
+#define VBG_BF16_HUGE_BK 32
+#endif
 
 template <int ROWS, int BK, bool KC>
 struct DmaPlane {
@@ -525,9 +529,9 @@ template <int BM, int BN, bool TA, bool TB, bool OUT_BF16>
 struct Bf16DmaTile {
   static constexpr bool HUGE = BM * BN >= 256 * 256;
   static constexpr bool BIG = BM * BN > 128 * 128;
-  static constexpr int BK = HUGE ? 64 : 32;
+  static constexpr int BK = HUGE ? VBG_BF16_HUGE_BK : 32;
   static constexpr int KS = BK / 16;  // MFMA k-steps per k-tile (even)
-  static constexpr int S = HUGE ? 2 : BIG ? 3 : 4;
+  static constexpr int S = HUGE ? 131072 / ((BM + BN) * BK * 2) : BIG ? 3 : 4;
   static constexpr int WAVES = HUGE ? 1 : 2;  // waves per SIMD
   static_assert(S >= 2 && S <= 4, "stage ring of 2 .. 4 buffers");
   using PA = DmaPlane<BM, BK, TA>;   // A is k-contiguous when transposed

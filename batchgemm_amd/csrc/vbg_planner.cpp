@@ -15,7 +15,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <numeric>
+#include <queue>
 
 #include "../../include/vbgemm.h"
 
@@ -66,6 +68,42 @@ int num_tiles(int tile, const Shape& s) {
 }
 
 double matrix_cost(int tile, const Shape& s) { return num_tiles(tile, s) * item_cost(tile, s); }
+
+// Modelled duration of a persistent launch: its items (cost, count) on
+// `slots` workgroup slots.  The workgroups pop cost-ordered queues, which is
+// greedy list scheduling longest-first; while a launch is only a few rounds
+// long that makespan is set by whole rounds (768 equal items on 256 slots take
+// 3 items' time, 769 take 4), so it is simulated.  Longer launches are
+// modelled as max(total / slots, longest item).
+struct CostCount {
+  double cost;
+  int64_t count;
+};
+double launch_makespan(std::vector<CostCount> items, int slots) {
+  const int kMaxRounds = 16;
+  double total = 0.0, longest = 0.0;
+  int64_t n = 0;
+  for (const CostCount& x : items) {
+    if (x.count <= 0) continue;
+    total += x.cost * (double)x.count;
+    longest = std::max(longest, x.cost);
+    n += x.count;
+  }
+  if (n == 0 || slots <= 0) return 0.0;
+  if (n > (int64_t)kMaxRounds * slots) return std::max(total / slots, longest);
+  std::sort(items.begin(), items.end(), [](const CostCount& a, const CostCount& b) { return a.cost > b.cost; });
+  std::priority_queue<double, std::vector<double>, std::greater<double>> load;
+  for (int i = 0; i < slots; ++i) load.push(0.0);
+  double span = 0.0;
+  for (const CostCount& x : items)
+    for (int64_t j = 0; j < x.count; ++j) {
+      const double l = load.top() + x.cost;
+      load.pop();
+      load.push(l);
+      span = std::max(span, l);
+    }
+  return span;
+}
 
 bool is_split(int prec) {
   return prec == VBG_PREC_BF16X3 || prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3 ||
@@ -342,9 +380,10 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
     hp.desc[b] = {M[b], N[b], K[b], lda[b], ldb[b], ldc[b], 0, 1};
   }
 
-  const int occ_default = 2;
+  // without a device (host-only plans): the kernels' workgroups per CU --
+  // two, one for the 256x256 tile (128 KB of LDS)
   auto occ = [&](int fam, int tile) {
-    int v = occupancy ? occupancy(fam, prec, tile, ta, tb) : occ_default;
+    int v = occupancy ? occupancy(fam, prec, tile, ta, tb) : (tile == T256 ? 1 : 2);
     return v > 0 ? v : 1;
   };
   const int cus = num_cus > 0 ? num_cus : 256;
@@ -456,7 +495,14 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
       longest[tile[b]] = std::max(longest[tile[b]], item_cost(tile[b], sh[b]));
     }
     auto slots = [&](int t) { return (double)cus * occ(fam, t); };
-    auto dur = [&](int t) { return std::max(tot[t] / slots(t), longest[t]); };
+    // items of each shape as (cost, count) per matrix; a launch's duration
+    // is the makespan model above
+    std::vector<std::vector<CostCount>> its(NTILES);
+    for (int b = 0; b < batch; ++b)
+      if (M[b] > 0 && N[b] > 0) its[tile[b]].push_back({item_cost(tile[b], sh[b]), num_tiles(tile[b], sh[b])});
+    double durs[NTILES] = {};
+    for (int t = 0; t < NTILES; ++t) durs[t] = launch_makespan(its[t], (int)slots(t));
+    auto dur = [&](int t) { return durs[t]; };
     int D = 0;
     for (int t = 1; t < NTILES; ++t)
       if (dur(t) > dur(D)) D = t;
@@ -470,20 +516,26 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
                                   dur(t) <= kLaunchFixed;
         if (!short_launch) continue;
         double add = 0.0, add_longest = 0.0;
+        std::vector<CostCount> joined = its[D];
         for (int b = 0; b < batch; ++b)
           if (tile[b] == t && M[b] > 0 && N[b] > 0) {
             add += matrix_cost(D, sh[b]);
             add_longest = std::max(add_longest, item_cost(D, sh[b]));
+            joined.push_back({item_cost(D, sh[b]), num_tiles(D, sh[b])});
           }
         const double before = dur(D);
-        const double after = std::max((tot[D] + add) / slots(D), std::max(longest[D], add_longest));
+        const double after = launch_makespan(joined, (int)slots(D));
         if (after - before < dur(t) + kLaunchFixed) {
           for (int b = 0; b < batch; ++b)
             if (tile[b] == t) tile[b] = D;
           tot[D] += add;
           longest[D] = std::max(longest[D], add_longest);
+          its[D].swap(joined);
+          durs[D] = after;
           tot[t] = 0.0;
           longest[t] = 0.0;
+          its[t].clear();
+          durs[t] = 0.0;
         }
       }
     }
@@ -498,10 +550,15 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   hp.ws_off.assign(batch, -1);
   if (prec != VBG_PREC_FP32_SEQ && o.split_k_min >= 0) {
     const int kmin = o.split_k_min > 0 ? o.split_k_min : 256;
-    double total = 0.0;
+    double total = 0.0, tot_t[NTILES] = {};
     for (int b = 0; b < batch; ++b)
-      if (M[b] > 0 && N[b] > 0) total += matrix_cost(tile[b], sh[b]);
-    const double avg = total / ((double)cus * occ(fam, T128));
+      if (M[b] > 0 && N[b] > 0) {
+        total += matrix_cost(tile[b], sh[b]);
+        tot_t[tile[b]] += matrix_cost(tile[b], sh[b]);
+      }
+    // slots of the shape carrying the most work
+    const int Dt = (int)(std::max_element(tot_t, tot_t + NTILES) - tot_t);
+    const double avg = total / ((double)cus * occ(fam, Dt));
     const int64_t ws_cap = ws_cap_floats;
     int64_t ws = 0;
     for (int b = 0; b < batch; ++b) {
@@ -513,6 +570,35 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
       } else if (K[b] >= kmin) {
         const double ic = item_cost(tile[b], sh[b]);
         if (ic > 1.25 * avg) S = std::min<int>({(int)std::ceil(2.0 * ic / avg), ks / 2, 16});
+        if (S >= 2) {
+          // The slice count that makes the launch of this matrix's shape
+          // shortest in whole rounds (the rule above can leave a last round
+          // nearly empty: one 1408 x 2048 x 32636 matrix on 256x256 tiles cut
+          // into 11 slices is 528 items on 256 slots, 16 slices 768): the
+          // other matrices of the shape as they stand, this one cut into
+          // S' = 2 .. the rule's cap.
+          const int cap = std::min(16, std::max(2, ks / 2));
+          std::vector<CostCount> others;
+          for (int c = 0; c < batch; ++c)
+            if (c != b && tile[c] == tile[b] && M[c] > 0 && N[c] > 0) {
+              const int Sc = hp.desc[c].ksplit;
+              others.push_back({kItemFixed + (item_cost(tile[c], sh[c]) - kItemFixed) / Sc,
+                                (int64_t)num_tiles(tile[c], sh[c]) * Sc});
+            }
+          const int nslots = cus * occ(fam, tile[b]);
+          double best = 0.0;
+          int bestS = S;
+          for (int s2 = 2; s2 <= cap; ++s2) {
+            std::vector<CostCount> its = others;
+            its.push_back({kItemFixed + (ic - kItemFixed) / s2, (int64_t)num_tiles(tile[b], sh[b]) * s2});
+            const double m = launch_makespan(its, nslots);
+            if (s2 == 2 || m < best - 1e-9) {
+              best = m;
+              bestS = s2;
+            }
+          }
+          S = bestS;
+        }
       }
       const int64_t need = (int64_t)S * M[b] * N[b];
       if (S >= 2 && ws + need <= ws_cap) {
