@@ -68,6 +68,8 @@ class vbg_opts(ctypes.Structure):
         ("mixed_tiles", ctypes.c_int),
         ("consolidate", ctypes.c_int),
         ("pack_operands", ctypes.c_int),
+        ("item_fixed_x10", ctypes.c_int),
+        ("small_tile_eff_pct", ctypes.c_int),
     ]
 
 

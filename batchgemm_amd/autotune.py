@@ -13,6 +13,10 @@ variants of this library's own cost model, are candidates:
                              edge tiles off / on; tile grouping off
 - ``auto`` + launch shape    one mixed launch for the 64..128 tiles, or one
                              launch per tile shape
+- ``auto`` + cost model      the cost model's constants on a grid: the fixed
+                             cost per work item (1, 3, 6, 12 units) times the
+                             modelled efficiency of the tiles below 128x128
+                             (x0.70 .. x1.50)
 - ``reference``              the reference planner, wavefront_thres 2500 / 6000 / 20000
 - ``wang`` / ``li``          TLP thresholds 65536 / 262144
 
@@ -68,6 +72,12 @@ CANDIDATES = [
     ("li:65536", {"policy": "li", "tlp_thres": 65536}),
     ("li:262144", {"policy": "li", "tlp_thres": 262144}),
 ]
+# The AUTO cost model's constants (the per-item fixed cost and the modelled
+# MFMA efficiency of the tiles below 128x128, which together set where the
+# planner trades large tiles for more, smaller items): a grid around the
+# defaults (3.0 units, x1.00).
+COST_GRID = [(f, e) for f in (10, 30, 60, 120) for e in (70, 85, 100, 120, 150) if (f, e) != (30, 100)]
+CANDIDATES += [(f"auto:cost:f{f}:e{e}", {"item_fixed_x10": f, "small_tile_eff_pct": e}) for f, e in COST_GRID]
 DEFAULT = "auto"
 
 

@@ -48,7 +48,23 @@ const int kTileBK[NTILES] = {64, 32, 32, 32, 32, 32, 16, 64, 64, 64};
 const double kTileEff[NTILES] = {0.20, 0.40, 0.70, 0.85, 0.85, 1.00, 1.10, 0.35, 0.35, 1.40};
 // Fixed cost of one work item in units of a 128x128x32 k-step (queue pop,
 // pipeline fill, epilogue).
-const double kItemFixed = 3.0;
+const double kItemFixedDefault = 3.0;
+
+// The cost model in force for the plan being built (the constants above, or
+// the values a plan's options set: the autotuner searches them).  Set at the
+// start of each planning call; thread-local, so plans built on different
+// threads do not interfere.
+struct CostModel {
+  double eff[NTILES];
+  double fixed;
+};
+thread_local CostModel g_cm;
+void set_cost_model(const vbg_opts& o) {
+  const double small = o.small_tile_eff_pct > 0 ? o.small_tile_eff_pct / 100.0 : 1.0;
+  for (int t = 0; t < NTILES; ++t)
+    g_cm.eff[t] = kTileEff[t] * (kTileBM[t] * kTileBN[t] < 128 * 128 ? small : 1.0);
+  g_cm.fixed = o.item_fixed_x10 > 0 ? o.item_fixed_x10 / 10.0 : kItemFixedDefault;
+}
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -60,7 +76,7 @@ double item_cost(int tile, const Shape& s) {
   const int bm = kTileBM[tile], bn = kTileBN[tile], bk = kTileBK[tile];
   const double ksteps = (double)cdiv(std::max(s.K, 1), bk);
   const double unit = 128.0 * 128.0 * 32.0;
-  return kItemFixed + ksteps * (double)bm * bn * bk / unit / kTileEff[tile];
+  return g_cm.fixed + ksteps * (double)bm * bn * bk / unit / g_cm.eff[tile];
 }
 
 int num_tiles(int tile, const Shape& s) {
@@ -330,6 +346,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
                int num_cus, int (*occupancy)(int, int, int, bool, bool), std::string& err,
                int64_t ws_cap_floats) {
   hp = HostPlan();
+  set_cost_model(o);
   hp.batch = batch;
   hp.ta = ta;
   hp.tb = tb;
@@ -582,7 +599,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
           for (int c = 0; c < batch; ++c)
             if (c != b && tile[c] == tile[b] && M[c] > 0 && N[c] > 0) {
               const int Sc = hp.desc[c].ksplit;
-              others.push_back({kItemFixed + (item_cost(tile[c], sh[c]) - kItemFixed) / Sc,
+              others.push_back({g_cm.fixed + (item_cost(tile[c], sh[c]) - g_cm.fixed) / Sc,
                                 (int64_t)num_tiles(tile[c], sh[c]) * Sc});
             }
           const int nslots = cus * occ(fam, tile[b]);
@@ -590,7 +607,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
           int bestS = S;
           for (int s2 = 2; s2 <= cap; ++s2) {
             std::vector<CostCount> its = others;
-            its.push_back({kItemFixed + (ic - kItemFixed) / s2, (int64_t)num_tiles(tile[b], sh[b]) * s2});
+            its.push_back({g_cm.fixed + (ic - g_cm.fixed) / s2, (int64_t)num_tiles(tile[b], sh[b]) * s2});
             const double m = launch_makespan(its, nslots);
             if (s2 == 2 || m < best - 1e-9) {
               best = m;
@@ -611,7 +628,7 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   }
   auto slice_cost = [&](int t, int b) {
     const int S = hp.desc[b].ksplit;
-    return kItemFixed + (item_cost(t, sh[b]) - kItemFixed) / S;
+    return g_cm.fixed + (item_cost(t, sh[b]) - g_cm.fixed) / S;
   };
 
   // 2. group matrices by kernel family/tile; 3. LPT over XCD queues.

@@ -127,6 +127,11 @@ typedef struct vbg_opts {
                            batch's arithmetic intensity, 2*sum(MNK) / 4*sum(MK+KN), is at
                            least 64 flop/B and the workspace stays within 8 GiB; bf16x3
                            unpacked) */
+  int item_fixed_x10;   /* AUTO cost model: fixed cost of one work item (queue pop, pipeline
+                           fill, epilogue) in tenths of a 128x128x32 k-step (0 = 30) */
+  int small_tile_eff_pct; /* AUTO cost model: scale (percent) of the modelled MFMA efficiency
+                           of the tiles smaller than 128x128 (0 = 100); above 100 favours
+                           them against the large tiles */
 } vbg_opts;
 
 typedef struct vbg_plan vbg_plan;

@@ -286,7 +286,7 @@ def test_opts_struct_matches_header(bg):
     src = open(os.path.join(ROOT, "include", "vbgemm.h")).read()
     body = re.search(r"typedef struct vbg_opts \{(.*?)\} vbg_opts;", src, re.S).group(1)
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
-    fields = re.findall(r"\b(?:int|unsigned)\s+([a-z_]+)\s*;", body)
+    fields = re.findall(r"\b(?:int|unsigned)\s+([a-z_0-9]+)\s*;", body)
     assert fields == [f for f, _ in bg._lib.vbg_opts._fields_]
     assert ctypes.sizeof(bg._lib.vbg_opts) == 4 * len(fields)
 
