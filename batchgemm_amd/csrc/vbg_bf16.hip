@@ -7,6 +7,7 @@
 // leading dimensions; the grouped entry points check the pointers).
 #include <utility>
 
+#include "vbg_dma.h"
 #include "vbg_launch.h"
 #include "vbg_tiles.h"
 #include "../../include/vbgemm.h"
@@ -373,368 +374,6 @@ struct Bf16Tile {
   }
 };
 
-// ------------------------------------------------------ LDS-DMA staged tiles
-// The operands go from HBM straight into LDS (buffer_load_dwordx4 ... lds:
-// lane l's 16 bytes land at the instruction's LDS base + 16 l), through a ring
-// of S stage buffers of BK = 32 k each, so S-1 k-tiles are in flight without
-// holding any of them in VGPRs (the register-staged Bf16Tile holds at most two
-// and spends VALU and LDS-write instructions on each).  Operands keep their
-// HBM orientation in LDS:
-//   KC (k contiguous): rows of 32 k (64 bytes, four 16-byte units, unit u of
-//   row r at u ^ ((r / 4) % 4)); an MFMA operand is one ds_read_b128.
-//   MC (mn contiguous): rows of one k over the tile's mn (2*ROWS bytes, in
-//   64-byte blocks, block b of k-row r at b ^ f(r)); an MFMA operand of 8 k is
-//   two ds_read_b64_tr_b16, the transposing LDS read of gfx950: within each
-//   16-lane group, lane 4i + q addresses 4 consecutive mn at k-row i, and
-//   lane j receives column j of that 4 x 16 block (mn j, 4 consecutive k).
-//   The swizzles put the rows that one read instruction touches in distinct
-//   LDS banks.  A DMA instruction fills 1 KB of consecutive LDS positions;
-//   each lane fetches the global 16 bytes that belong at its position.
-// Rows past MN read the last row (KC) or last 8-element unit (MC): they only
-// feed outputs that are never stored.  The k-tile that crosses the end of the
-// item's k range puts its whole offset in the range-checked vector offset and
-// sends each 16-byte unit that starts at k >= end past the buffer's extent
-// (no memory access); once it has landed, each lane zeroes its units at
-// k >= end in LDS, element by element in a KC unit that straddles the end
-// (K % 8 != 0).
-//   Waits: each k-tile's barrier is preceded by vmcnt(N), N = the DMAs issued
-// after that tile's (see vbg_dma16).
-#ifndef VBG_BF16_DMA
-#define VBG_BF16_DMA 1
-#endif
-// k depth of the 256x256 tile's stages: 64 (a ring of 2 x 64 KB) or 32 (4 x 32 KB)
-#ifndef VBG_BF16_HUGE_BK
-#define VBG_BF16_HUGE_BK 32
-#endif
-
-template <int ROWS, int BK, bool KC>
-struct DmaPlane {
-  static_assert(BK == 32 || BK == 64, "k-tile depth 32 or 64");
-  static constexpr int BYTES = ROWS * BK * 2;
-  static constexpr int PITCH = KC ? BK * 2 : ROWS * 2;  // bytes per LDS row
-  static constexpr int NU = PITCH / 16;                 // KC: 16-byte units per row
-  static constexpr int RPB = 256 / PITCH;               // KC: rows per 256-byte bank line
-  static constexpr int NB = PITCH / 64;                 // MC: 64-byte blocks per k-row
-  static_assert(KC || NB == 2 || NB % 4 == 0, "MC rows of 2 or a multiple of 4 blocks");
-  __device__ __forceinline__ static int kc_unit(int row, int u) { return (u ^ (row / RPB)) & (NU - 1); }
-  __device__ __forceinline__ static int mc_f(int r) { return NB >= 4 ? (r & 3) : ((r >> 1) & 1); }
-
-  // The 16 bytes lane `lane` of wave `wave` fetches for this plane's chunk
-  // 4 i + wave (a chunk is 1 KB of consecutive LDS positions): its mn row
-  // (KC) or k-row (MC) advances by a fixed step with i while its unit or
-  // block stays the same (the swizzles repeat every 4 KB), so a lane keeps
-  // one origin per plane: KC row r0 + i * RSTEP at byte u16 within the row;
-  // MC byte offset base0 + i * KSTEP * ld * 2, k-row kk0 + i * KSTEP.
-  static constexpr int RSTEP = KC ? 4096 / PITCH : 0;
-  static constexpr int KSTEP = KC ? 0 : 4096 / PITCH;
-  struct Src {
-    int r0, u16;  // KC: tile row of chunk 0, byte of the unit within the row
-    int base0;    // MC: byte offset of chunk 0 (k-tile at k = 0)
-    int kk0;      // first k of chunk 0's unit within the k-tile
-  };
-  __device__ __forceinline__ static Src src(int wave, int lane, int mn0, int MN, int ld) {
-    const int q = wave * 1024 + 16 * lane;
-    Src o{};
-    if (KC) {
-      const int row = q / PITCH, unit = kc_unit(row, (q % PITCH) >> 4);
-      o.r0 = mn0 + row;
-      o.u16 = 16 * unit;
-      o.kk0 = 8 * unit;
-    } else {
-      const int krow = q / PITCH, pb = (q % PITCH) >> 6, within = q & 63;
-      const int lb = pb ^ mc_f(krow);
-      const int mn = min(mn0 + 32 * lb + (within >> 1), (MN - 1) & ~7);
-      o.base0 = (mn + krow * ld) * 2;
-      o.kk0 = krow;
-    }
-    return o;
-  }
-  // Byte offset (k-tile at k = 0) of chunk i: KC rows past MN read row MN - 1
-  // (they only feed outputs that are never stored), so no offset leaves the
-  // operand whatever k offset is added within a whole tile.
-  __device__ __forceinline__ static int off(const Src& o, int i, int MN, int ld) {
-    if (KC) return min(o.r0 + i * RSTEP, MN - 1) * (2 * ld) + o.u16;
-    return o.base0 + i * KSTEP * (2 * ld);
-  }
-  __device__ __forceinline__ static int kk(const Src& o, int i) { return o.kk0 + i * KSTEP; }
-  // bytes of one k step in HBM
-  __device__ __forceinline__ static int kstride(int ld) { return KC ? 2 : 2 * ld; }
-  // Operand extent in bytes: KC MN rows of K at ld, MC K rows of MN at ld,
-  // the last row rounded up to whole 16-byte units (the range check works per
-  // dword and would zero valid elements that share a dword with the end).
-  // With ld % 8 == 0 and a 16-byte-aligned base, that unit lies in the
-  // 16-byte block of the operand's last element.
-  __device__ __forceinline__ static uint32_t extent(int MN, int K, int ld) {
-    if (MN <= 0 || K <= 0) return 0u;
-    return KC ? ((uint32_t)(MN - 1) * (uint32_t)ld + (uint32_t)((K + 7) & ~7)) * 2u
-              : ((uint32_t)(K - 1) * (uint32_t)ld + (uint32_t)((MN + 7) & ~7)) * 2u;
-  }
-  // The MFMA operand of fragment rows fb .. fb+31 (fb % 32 == 0) at k16 step
-  // s: lane (l32, h) gets mn fb + l32, k 16 s + 8 h + [0, 8).
-  __device__ __forceinline__ static v8bf read(const char* plane, int fb, int s, int lane) {
-    if (KC) {
-      const int row = fb + (lane & 31), unit = 2 * s + (lane >> 5);
-      return *(const v8bf*)(plane + row * PITCH + (kc_unit(row, unit) << 4));
-    } else {
-      typedef __bf16 v4bf_t __attribute__((ext_vector_type(4)));
-      typedef __attribute__((address_space(3))) v4bf_t* LP;
-      const int G = lane >> 4, j = lane & 15;
-      const int kr = 16 * s + 8 * (G >> 1) + (j >> 2);
-      const int lb = fb >> 5, off = 32 * (G & 1) + 8 * (j & 3);
-      const char* p0 = plane + kr * PITCH + ((lb ^ mc_f(kr)) << 6) + off;
-      const char* p1 = plane + (kr + 4) * PITCH + ((lb ^ mc_f(kr + 4)) << 6) + off;
-      const v4bf_t x0 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((LP)(p0));
-      const v4bf_t x1 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((LP)(p1));
-      return __builtin_shufflevector(x0, x1, 0, 1, 2, 3, 4, 5, 6, 7);
-    }
-  }
-};
-
-// f(integral_constant<int, j>) for j = 0 .. N-1, in order.
-template <class F, int... J>
-__device__ __forceinline__ void vbg_static_for_impl(F&& f, std::integer_sequence<int, J...>) {
-  (f(std::integral_constant<int, J>{}), ...);
-}
-template <int N, class F>
-__device__ __forceinline__ void vbg_static_for(F&& f) {
-  vbg_static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-// s_waitcnt vmcnt(n * NI) + barrier for a run-time n in [0, MAXN].
-template <int NI, int MAXN>
-__device__ __forceinline__ void vbg_wait_tiles(int n) {
-  if constexpr (MAXN > 0) {
-    if (n >= MAXN) {
-      vbg_wait_vm_barrier<MAXN * NI>();
-      return;
-    }
-    vbg_wait_tiles<NI, MAXN - 1>(n);
-  } else {
-    vbg_wait_vm_barrier<0>();
-  }
-}
-
-// Shapes.  The bytes a CU can move from HBM/L2 into LDS per cycle through the
-// buffer-load address path, not the MFMA rate, bound the smaller tiles (a
-// 128x128 tile needs about as many bytes per MFMA cycle as that path
-// delivers), so the large shapes carry the big matrices:
-//   256x256: 4 waves of 128 x 128 (one per SIMD, 256 accumulators each), 64-deep
-//            k-tiles (operands stored k-contiguous fetch whole 128-byte lines),
-//            a ring of 2 x 64 KB, one workgroup per CU;
-//   256x128: 4 waves of 128 x 64, 32-deep k-tiles, a ring of 3 x 24 KB, two
-//            workgroups per CU;
-//   64x64 .. 128x128: 4 waves of up to 64 x 64, 32-deep k-tiles, a ring of
-//            4 x <= 16 KB, two workgroups per CU.
-template <int BM, int BN, bool TA, bool TB, bool OUT_BF16>
-struct Bf16DmaTile {
-  static constexpr bool HUGE = BM * BN >= 256 * 256;
-  static constexpr bool BIG = BM * BN > 128 * 128;
-  static constexpr int BK = HUGE ? VBG_BF16_HUGE_BK : 32;
-  static constexpr int KS = BK / 16;  // MFMA k-steps per k-tile (even)
-  static constexpr int S = HUGE ? 131072 / ((BM + BN) * BK * 2) : BIG ? 3 : 4;
-  static constexpr int WAVES = HUGE ? 1 : 2;  // waves per SIMD
-  static_assert(S >= 2 && S <= 4, "stage ring of 2 .. 4 buffers");
-  using PA = DmaPlane<BM, BK, TA>;   // A is k-contiguous when transposed
-  using PB = DmaPlane<BN, BK, !TB>;  // B is k-contiguous when not transposed
-  static constexpr int PLANE_A = PA::BYTES, PLANE_B = PB::BYTES;
-  static constexpr int STAGE = PLANE_A + PLANE_B;
-  static constexpr int LDS_BYTES = S * STAGE;
-  static constexpr int CA = PLANE_A / 1024;    // 1 KB chunks of A per stage
-  static constexpr int NI = STAGE / 1024 / 4;  // DMA instructions per wave per stage
-  static_assert(STAGE % 4096 == 0 && CA % 4 == 0, "whole 1 KB chunks, four per wave row");
-  static_assert(NI * (S - 1) < 64, "vmcnt range");
-  static constexpr int WTM = BM / 2, WTN = BN / 2;
-  static constexpr int TM = WTM / 32, TN = WTN / 32;
-  static_assert(TM >= 1 && TN >= 1, "tile needs >= 64x64");
-  struct Frags {
-    v8bf a[TM], b[TN];
-  };
-
-  __device__ __forceinline__ static void run(const vbg_args& a, const vbg_item& it, char* lds) {
-    const vbg_mat_desc d = vbg_udesc(vbg_cload(a.desc, it.mat));
-    const void* A = vbg_uptr(vbg_cload(a.A, it.mat));
-    const void* B = vbg_uptr(vbg_cload(a.B, it.mat));
-    const int m0 = it.tm * BM, n0 = it.tn * BN;
-    const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
-    const int kb = kr.kb, ke = kr.ke;
-    const int nk = (ke - kb + BK - 1) / BK;
-    const int lane = threadIdx.x & 63, wave = vbg_u(threadIdx.x >> 6);
-    const int wr = wave & 1, wc = wave >> 1;
-    const int l32 = lane & 31, h = lane >> 5;
-
-    // DMA sources: chunk 4 i + wave of each stage is this wave's i-th (the
-    // first CA chunks are A's), so whether an instruction fills A or B is
-    // known at compile time.
-    const v4u32_t rsA = vbg_rsrc_words(A, PA::extent(d.M, d.K, d.lda));
-    const v4u32_t rsB = vbg_rsrc_words(B, PB::extent(d.N, d.K, d.ldb));
-    const int ksA = PA::kstride(d.lda), ksB = PB::kstride(d.ldb);
-    const typename PA::Src sa = PA::src(wave, lane, m0, d.M, d.lda);
-    const typename PB::Src sb = PB::src(wave, lane, n0, d.N, d.ldb);
-    constexpr int IA = CA / 4;  // this wave's chunks of A per stage
-    auto off = [&](int i) { return i < IA ? PA::off(sa, i, d.M, d.lda) : PB::off(sb, i - IA, d.N, d.ldb); };
-    auto kk = [&](int i) { return i < IA ? PA::kk(sa, i) : PB::kk(sb, i - IA); };
-    auto stage = [&](int j) { return lds + j * STAGE; };
-    auto dma = [&](char* stg, int t) {
-      const int k0 = kb + t * BK;
-      const uint32_t lbase = (uint32_t)(uintptr_t)stg + (uint32_t)(wave * 1024);
-      if (k0 + BK <= ke) {  // wave-uniform
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-          if (i < IA) vbg_dma16(rsA, vbg_u(lbase + 4096 * i), off(i), vbg_u(k0 * ksA));
-          else vbg_dma16(rsB, vbg_u(lbase + 4096 * i), off(i), vbg_u(k0 * ksB));
-        }
-      } else {
-        // the whole offset in the range-checked vector offset; units at
-        // k >= end go past the extent (no access)
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-          const int ks = i < IA ? ksA : ksB;
-          const int v = kk(i) < ke - k0 ? off(i) + k0 * ks : (int)0xfffffff0u;
-          vbg_dma16(i < IA ? rsA : rsB, vbg_u(lbase + 4096 * i), v, 0);
-        }
-      }
-    };
-    // Wait for tile t (n tiles issued after it), and when it crosses the end
-    // of the k range zero what this lane's DMAs placed at k >= end, element by
-    // element in a KC unit that straddles the end (K % 8 != 0).
-    auto arrive = [&](char* stg, int t, int n) {
-      vbg_wait_tiles<NI, S - 1>(n);
-      const int k0 = kb + t * BK;
-      if (k0 + BK <= ke) return;  // wave-uniform
-      const int rel = ke - k0;
-      char* mine = stg + wave * 1024 + 16 * lane;
-#pragma unroll
-      for (int i = 0; i < NI; ++i) {
-        const bool kc = i < IA ? TA : !TB;
-        v4u* p = (v4u*)(mine + 4096 * i);
-        if (kk(i) >= rel) {
-          *p = v4u{0u, 0u, 0u, 0u};
-        } else if (kc && kk(i) + 8 > rel) {
-          const int lim = rel - kk(i);  // elements of this unit below the end
-          v4u x = *p;
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            x[e] &= (2 * e < lim ? 0xffffu : 0u) | (2 * e + 1 < lim ? 0xffff0000u : 0u);
-          *p = x;
-        }
-      }
-      vbg_wait_vm_barrier<0>();
-    };
-
-    v16f acc[TM][TN];
-#pragma unroll
-    for (int r = 0; r < TM; ++r)
-#pragma unroll
-      for (int c = 0; c < TN; ++c)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) acc[r][c][e] = 0.0f;
-    auto read = [&](Frags& f, const char* stg, int s) {
-#pragma unroll
-      for (int r = 0; r < TM; ++r) f.a[r] = PA::read(stg, wr * WTM + r * 32, s, lane);
-#pragma unroll
-      for (int c = 0; c < TN; ++c) f.b[c] = PB::read(stg + PLANE_A, wc * WTN + c * 32, s, lane);
-    };
-    // MFMAs of one k16 step; FROM skips the first `from` (issued earlier)
-    auto mma = [&](const Frags& f, int from = 0) {
-#pragma unroll
-      for (int r = 0; r < TM; ++r)
-#pragma unroll
-        for (int c = 0; c < TN; ++c)
-          if (r * TN + c >= from)
-            acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[r], f.b[c], acc[r][c], 0, 0, 0);
-    };
-    auto mma_first = [&](const Frags& f) {
-      acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[0], f.b[0], acc[0][0], 0, 0, 0);
-    };
-
-    // Pipeline.  Tile t sits in stage t % S.  The MFMA operands of each k16
-    // step are read from LDS one step ahead, into the other register set, so
-    // the LDS latency overlaps the previous step's MFMAs.  Moving to tile
-    // t + 1: wait for its DMAs (and, via lgkmcnt(0), for this wave's reads of
-    // tile t), meet at the barrier -- after which no wave reads stage t % S
-    // again -- refill that stage with tile t + S, then read tile t + 1's
-    // first step.  S - 1 tiles are in flight ahead of the MFMAs.
-    // FAST: the caller knows tile t + 1 is a whole tile with at least S - 2
-    // tiles after it (the branch-free steady state).
-    auto next = [&](auto I, auto FAST, int t, Frags& f) {
-      constexpr int i = decltype(I)::value;
-      if constexpr (decltype(FAST)::value) vbg_wait_vm_barrier<(S - 2) * NI>();
-      else arrive(stage((i + 1) % S), t + 1, min(S - 2, nk - 2 - t));
-      if (t + S < nk) dma(stage(i), t + S);
-      read(f, stage((i + 1) % S), 0);
-    };
-    // One k-tile: KS k16 steps, alternating between the two register sets
-    // (KS is even, so every tile starts in f0).  Each step issues its first
-    // MFMA, then the reads of the next step (the next tile's first step,
-    // after the tile transition, in the last), then its other MFMAs.
-    // Scheduling fences keep each phase whole.  (The first MFMA goes before
-    // the reads because the compiler waits for every outstanding LDS read
-    // before the first use of a transposing read's result, so reads issued
-    // just before that use would stall it.)
-    Frags f0, f1;
-    auto body = [&](auto I, auto FAST, int t) {
-      constexpr bool fast = decltype(FAST)::value;
-      vbg_static_for<KS>([&](auto SS) {
-        constexpr int s = decltype(SS)::value;
-        Frags& cur = s % 2 == 0 ? f0 : f1;
-        Frags& nxt = s % 2 == 0 ? f1 : f0;
-        mma_first(cur);
-        __builtin_amdgcn_sched_barrier(0);
-        if constexpr (s + 1 < KS) read(nxt, stage(decltype(I)::value), s + 1);
-        else if (fast || t + 1 < nk) next(I, FAST, t, nxt);
-        __builtin_amdgcn_sched_barrier(0);
-        mma(cur, 1);
-        __builtin_amdgcn_sched_barrier(0);
-      });
-    };
-#pragma unroll
-    for (int j = 0; j < S; ++j)
-      if (j < nk) dma(stage(j), j);
-    if (nk > 0) {
-      arrive(stage(0), 0, min(S - 1, nk - 1));
-      read(f0, stage(0), 0);
-    }
-    // Steady state (unrolled over the ring): every tile of the group has a
-    // successor that is a whole tile with S - 2 tiles issued after it, so the
-    // group runs without branches around its waits.
-    const int nwhole = (ke - kb) / BK;
-    int t = 0;
-    for (; t + S + S - 2 < nwhole; t += S)
-      vbg_static_for<S>([&](auto J) { body(J, std::true_type{}, t + decltype(J)::value); });
-    for (; t + S <= nk; t += S)
-      vbg_static_for<S>([&](auto J) { body(J, std::false_type{}, t + decltype(J)::value); });
-    vbg_static_for<S - 1>([&](auto J) {
-      if (t + decltype(J)::value < nk) body(J, std::false_type{}, t + decltype(J)::value);
-    });
-    // no stage is refilled by the next item before every wave has read this
-    // item's last tile
-    vbg_wait_vm_barrier<0>();
-
-    const OutTile o = vbg_out(a, d, it, 1.0f);
-#pragma unroll
-    for (int r = 0; r < TM; ++r)
-#pragma unroll
-      for (int c = 0; c < TN; ++c)
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
-          const int col = n0 + wc * WTN + c * 32 + l32;
-          v4f v = {acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
-          if (OUT_BF16 && !o.partial)
-            store4_bf16((u16*)o.p, o.ld, row, col, d.M, d.N, v, o.alpha, o.beta);
-          else
-            vbg_store4(o.p, o.ld, row, col, d.M, d.N, v, o.alpha, o.beta);
-        }
-  }
-};
-
-template <class Tile>
-__global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(Tile::WAVES)))
-vbg_bf16_dma_kernel(vbg_args a) {
-  __shared__ __attribute__((aligned(1024))) char lds[Tile::LDS_BYTES];
-  vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
-}
-
 // Two workgroups per CU (the 64 KB LDS of the 128x128 tile allows no more):
 // the register allocation is held to two waves per SIMD.
 template <class Tile>
@@ -746,28 +385,30 @@ vbg_bf16_kernel(vbg_args a) {
 
 }  // namespace
 
-template <class Tile, bool DMA>
+template <class Tile>
 static int run_or_occ(const vbg_args* a, int grid, hipStream_t s, int* occ) {
-  auto kern = [] {
-    if constexpr (DMA) return vbg_bf16_dma_kernel<Tile>;
-    else return vbg_bf16_kernel<Tile>;
-  }();
   if (occ) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, VBG_THREADS, 0) != hipSuccess) nb = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, vbg_bf16_kernel<Tile>, VBG_THREADS, 0) != hipSuccess)
+      nb = 1;
     *occ = nb > 0 ? nb : 1;
     return 0;
   }
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(VBG_THREADS), 0, s, *a);
+  hipLaunchKernelGGL(vbg_bf16_kernel<Tile>, dim3(grid), dim3(VBG_THREADS), 0, s, *a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+// The LDS-DMA tiles (vbg_dma.h) by default; the register-staged Bf16Tile
+// with VBG_BF16_DMA=0 (shapes up to 128x128).
+#ifndef VBG_BF16_DMA
+#define VBG_BF16_DMA 1
+#endif
 template <int BM, int BN, bool TA, bool TB, bool OB>
 static int run_tile(const vbg_args* a, int grid, hipStream_t s, int* occ) {
   if constexpr (VBG_BF16_DMA || BM * BN > 128 * 128)
-    return run_or_occ<Bf16DmaTile<BM, BN, TA, TB, OB>, true>(a, grid, s, occ);
+    return vbg_run_dmatile<DmaTile<BM, BN, TA, TB, OB ? DM_BF16_OUT : DM_BF16>>(a, grid, s, occ);
   else
-    return run_or_occ<Bf16Tile<BM, BN, TA, TB, OB>, false>(a, grid, s, occ);
+    return run_or_occ<Bf16Tile<BM, BN, TA, TB, OB>>(a, grid, s, occ);
 }
 
 template <int BM, int BN, bool OB>

@@ -57,6 +57,7 @@ const double kItemFixedDefault = 3.0;
 struct CostModel {
   double eff[NTILES];
   double fixed;
+  bool packed;  // split precisions: operands packed (the 256x256 tile needs them)
 };
 thread_local CostModel g_cm;
 void set_cost_model(const vbg_opts& o) {
@@ -64,6 +65,7 @@ void set_cost_model(const vbg_opts& o) {
   for (int t = 0; t < NTILES; ++t)
     g_cm.eff[t] = kTileEff[t] * (kTileBM[t] * kTileBN[t] < 128 * 128 ? small : 1.0);
   g_cm.fixed = o.item_fixed_x10 > 0 ? o.item_fixed_x10 / 10.0 : kItemFixedDefault;
+  g_cm.packed = false;
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -137,8 +139,11 @@ bool tile_allowed(int prec, int tile, unsigned mask) {
   if (mask && !(mask & (1u << tile))) return false;
   if (is_split(prec) && tile < T64) return false;
   const bool bf16 = prec == VBG_PREC_BF16 || prec == VBG_PREC_BF16_OUT_BF16;
+  // (packed FP8's 256x256 body, two sets of 32-byte f8f6f4 operands beside 256
+  // accumulators, does not fit the register file)
+  const bool pk_dma = g_cm.packed && prec == VBG_PREC_FP16_SCALED;
   if (tile == T256x128 && prec != VBG_PREC_BF16X3 && !bf16) return false;
-  if (tile == T256 && !bf16) return false;
+  if (tile == T256 && !bf16 && !pk_dma) return false;
   if ((tile == T16x128 || tile == T128x16) && prec != VBG_PREC_FP32) return false;
   return true;
 }
@@ -405,6 +410,37 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   };
   const int cus = num_cus > 0 ? num_cus : 256;
 
+  // Packed operands for bf16x3 / FP16 / FP8: one pre-pass quantises every
+  // operand into k-contiguous rows (4 / 2 / 1 bytes per element), which the
+  // GEMM kernels then stream with no conversion on the HBM -> LDS path.  The
+  // pass costs a read and a write of the operands (about 4.3 TB/s), so by
+  // default an FP16 / FP8 batch packs when its FP32 arithmetic intensity,
+  // 2*sum(MNK) / 4*sum(MK + KN) flop/B, reaches kPackMinAI.  Measured, ms
+  // (unpacked / packed incl. the pass): unbalanced B=2048 (AI ~156) FP16
+  // 3.85 / 3.57, FP8 3.86 / 3.18; log-uniform 8..1024, B=4096 (AI ~14) FP16
+  // 1.25 / 1.55, FP8 1.26 / 1.43.  bf16x3 stays unpacked by default: its
+  // GEMM kernel runs no faster on packed operands (3.80 / 3.95 ms on the
+  // unbalanced batch; the conversion is not what bounds it), so the pass
+  // (1.26 ms) is pure cost.
+  const bool packable = prec == VBG_PREC_BF16X3 || prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3;
+  bool pack = packable && o.pack_operands > 0;
+  // The default also leaves a batch unpacked when its packed operands would
+  // exceed kPackMaxBytes of plan-owned memory (asking for packing explicitly
+  // has no limit beyond the 2 GiB per operand the kernels address).
+  if (packable && o.pack_operands == 0 && prec != VBG_PREC_BF16X3) {
+    const double kPackMinAI = 64.0, kPackMaxBytes = 8.0 * (1u << 30);
+    const double ebp = prec == VBG_PREC_FP16_SCALED ? 2.0 : 1.0;
+    double flops = 0.0, bytes = 0.0, packed = 0.0;
+    for (int b = 0; b < batch; ++b) {
+      if (M[b] == 0 || N[b] == 0 || K[b] == 0) continue;
+      flops += 2.0 * M[b] * N[b] * K[b];
+      bytes += 4.0 * ((double)M[b] * K[b] + (double)K[b] * N[b]);
+      packed += ebp * ((double)M[b] + N[b]) * (double)(cdiv(K[b], VBG_PACK_KALIGN) * VBG_PACK_KALIGN);
+    }
+    pack = bytes > 0.0 && flops / bytes >= kPackMinAI && packed <= kPackMaxBytes;
+  }
+  g_cm.packed = pack;
+
   // 1. tile per matrix
   std::vector<int> tile(batch, T64);
   if (prec == VBG_PREC_FP32_SEQ) {
@@ -420,7 +456,8 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
           return VBG_ERR_INVALID_ARG;
         }
         if ((o.fixed_tile == T256x128 || o.fixed_tile == T256) && !tile_allowed(prec, o.fixed_tile, 0)) {
-          err = "the 256x128 tile is built for bf16x3 and BF16 operands, 256x256 for BF16 only";
+          err = "the 256x128 tile is built for bf16x3 and BF16 operands, 256x256 for BF16 and packed "
+                "FP16 / FP8 operands only";
           return VBG_ERR_NOT_SUPPORTED;
         }
         if ((o.fixed_tile == T16x128 || o.fixed_tile == T128x16) && prec != VBG_PREC_FP32) {
@@ -457,35 +494,6 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   if (is_split(prec))
     for (int b = 0; b < batch; ++b) tile[b] = std::max(tile[b], (int)T64);
   hp.tile_of.assign(batch, -1);
-  // Packed operands for bf16x3 / FP16 / FP8: one pre-pass quantises every
-  // operand into k-contiguous rows (4 / 2 / 1 bytes per element), which the
-  // GEMM kernels then stream with no conversion on the HBM -> LDS path.  The
-  // pass costs a read and a write of the operands (about 4.3 TB/s), so by
-  // default an FP16 / FP8 batch packs when its FP32 arithmetic intensity,
-  // 2*sum(MNK) / 4*sum(MK + KN) flop/B, reaches kPackMinAI.  Measured, ms
-  // (unpacked / packed incl. the pass): unbalanced B=2048 (AI ~156) FP16
-  // 3.85 / 3.57, FP8 3.86 / 3.18; log-uniform 8..1024, B=4096 (AI ~14) FP16
-  // 1.25 / 1.55, FP8 1.26 / 1.43.  bf16x3 stays unpacked by default: its
-  // GEMM kernel runs no faster on packed operands (3.80 / 3.95 ms on the
-  // unbalanced batch; the conversion is not what bounds it), so the pass
-  // (1.26 ms) is pure cost.
-  const bool packable = prec == VBG_PREC_BF16X3 || prec == VBG_PREC_FP16_SCALED || prec == VBG_PREC_FP8_E4M3;
-  bool pack = packable && o.pack_operands > 0;
-  // The default also leaves a batch unpacked when its packed operands would
-  // exceed kPackMaxBytes of plan-owned memory (asking for packing explicitly
-  // has no limit beyond the 2 GiB per operand the kernels address).
-  if (packable && o.pack_operands == 0 && prec != VBG_PREC_BF16X3) {
-    const double kPackMinAI = 64.0, kPackMaxBytes = 8.0 * (1u << 30);
-    const double ebp = prec == VBG_PREC_FP16_SCALED ? 2.0 : 1.0;
-    double flops = 0.0, bytes = 0.0, packed = 0.0;
-    for (int b = 0; b < batch; ++b) {
-      if (M[b] == 0 || N[b] == 0 || K[b] == 0) continue;
-      flops += 2.0 * M[b] * N[b] * K[b];
-      bytes += 4.0 * ((double)M[b] * K[b] + (double)K[b] * N[b]);
-      packed += ebp * ((double)M[b] + N[b]) * (double)(cdiv(K[b], VBG_PACK_KALIGN) * VBG_PACK_KALIGN);
-    }
-    pack = bytes > 0.0 && flops / bytes >= kPackMinAI && packed <= kPackMaxBytes;
-  }
   const int fam = prec == VBG_PREC_FP32_SEQ ? FAM_SEQ
                   : prec == VBG_PREC_FP32   ? FAM_FP32
                   : (prec == VBG_PREC_BF16 || prec == VBG_PREC_BF16_OUT_BF16) ? FAM_BF16

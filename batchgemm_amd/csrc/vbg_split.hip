@@ -11,6 +11,7 @@
 // work map and persistent driver are shared with the FP32 family.
 #include <type_traits>
 
+#include "vbg_dma.h"
 #include "vbg_launch.h"
 #include "vbg_tiles.h"
 #include "../../include/vbgemm.h"
@@ -511,6 +512,8 @@ vbg_split_kernel(vbg_args a) {
 #ifndef VBG_DMA_STAGES
 #define VBG_DMA_STAGES 4
 #endif
+// Packed FP16 / FP8 GEMM kernels: 1 = the shared LDS-DMA tiles (vbg_dma.h,
+// with the 256x256 shape), 2 = the LDS-DMA tiles below, 0 = register staging.
 #ifndef VBG_PK_DMA
 #define VBG_PK_DMA 1
 #endif
@@ -969,13 +972,29 @@ static int run_or_occ_dma(const vbg_args* a, int grid, hipStream_t s, int* occ) 
 // staging / DMA ring: FP16 2.19 / 2.19, FP8 1.85 / 1.61, bf16x3 3.53 / 4.82.
 template <int MODE>
 static int dispatch_tile_pk(int tile, const vbg_args* a, int grid, hipStream_t s, int* occ) {
-  if constexpr (VBG_PK_DMA && MODE != MODE_BF16X3) {
+  if constexpr (VBG_PK_DMA == 2 && MODE != MODE_BF16X3) {
     switch (tile) {
       case T64: return run_or_occ_dma<DmaPkTile<64, 64, MODE>>(a, grid, s, occ);
       case T128x64: return run_or_occ_dma<DmaPkTile<128, 64, MODE>>(a, grid, s, occ);
       case T64x128: return run_or_occ_dma<DmaPkTile<64, 128, MODE>>(a, grid, s, occ);
       case T128: return run_or_occ_dma<DmaPkTile<128, 128, MODE>>(a, grid, s, occ);
       case T_MIXED: return run_or_occ_dma<DmaPkMixed<MODE>>(a, grid, s, occ);
+      default: return -1;
+    }
+  }
+  if constexpr (VBG_PK_DMA == 1 && MODE != MODE_BF16X3) {
+    // the shared LDS-DMA tiles (vbg_dma.h): FP16 on the 32x32x16 MFMA, FP8 on
+    // the 32x32x64 f8f6f4 MFMA
+    constexpr int DM = MODE == MODE_FP16 ? DM_PK_FP16 : DM_PK_FP8;
+    switch (tile) {
+      case T64: return vbg_run_dmatile<DmaTile<64, 64, true, false, DM>>(a, grid, s, occ);
+      case T128x64: return vbg_run_dmatile<DmaTile<128, 64, true, false, DM>>(a, grid, s, occ);
+      case T64x128: return vbg_run_dmatile<DmaTile<64, 128, true, false, DM>>(a, grid, s, occ);
+      case T128: return vbg_run_dmatile<DmaTile<128, 128, true, false, DM>>(a, grid, s, occ);
+      case T256:
+        if constexpr (DM == DM_PK_FP16) return vbg_run_dmatile<DmaTile<256, 256, true, false, DM>>(a, grid, s, occ);
+        return -1;
+      case T_MIXED: return vbg_run_dmatile<DmaTileMixed<DM>>(a, grid, s, occ);
       default: return -1;
     }
   }

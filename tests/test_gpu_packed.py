@@ -3,10 +3,12 @@
 The pack pre-pass quantises A and B once per run into k-contiguous rows and the
 GEMM kernels stream those rows; it applies the same per-element rounding (bf16
 hi/lo split, x * 2^e to FP16, x * 448/amax to e4m3) as the in-loop
-quantisation, and the MFMA sequence per output element is unchanged.  So a
-packed run must equal the unpacked run bit for bit -- and the unpacked run is
-checked against the FP64 oracle and a numpy emulation of the quantisation in
-test_gpu_precision.py.  Cases: every transpose pair, ragged M / N / K (K not a
+quantisation, and for bf16x3 and FP16 the MFMA sequence per output element is
+unchanged.  So a packed run must equal the unpacked run bit for bit -- and the
+unpacked run is checked against the FP64 oracle and a numpy emulation of the
+quantisation in test_gpu_precision.py.  Packed FP8 runs on the 32x32x64
+f8f6f4 MFMA (the in-loop path on the 32x32x16 FP8 one), which accumulates
+differently: it is held to the FP8 accumulation bound instead.  Cases: every transpose pair, ragged M / N / K (K not a
 multiple of the 64-element row padding, K = 0, M or N = 1), several pack jobs
 per operand (M, N > 64 and K > 256), leading dimensions past the row count
 whose padding holds 1e30 (a pack that read it would blow the result up),
@@ -71,7 +73,17 @@ def test_packed_equals_unpacked(bg, gpu, prec, trans):
                 torch.cuda.synchronize()
                 out[pk] = vb.C.clone()
             assert torch.isfinite(out[1]).all()
-            assert torch.equal(out[1].view(torch.int32), out[-1].view(torch.int32)), seed
+            if prec == "fp8":
+                # Packed FP8 runs on the 32x32x64 f8f6f4 MFMA, the in-loop
+                # path on the 32x32x16 FP8 one: the same quantised operands,
+                # but the two accumulate their products differently inside
+                # the MFMA (each within the FP8 accumulation bound against
+                # exact products, checked in test_gpu_precision.py and
+                # test_gpu_workloads.py).  Here: equal to 2 x 2e-4 relative.
+                d = float(torch.linalg.norm((out[1] - out[-1]).double()))
+                assert d <= 4e-4 * float(torch.linalg.norm(out[-1].double())), (seed, d)
+            else:
+                assert torch.equal(out[1].view(torch.int32), out[-1].view(torch.int32)), seed
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "fp8"])

@@ -177,9 +177,13 @@ def test_mixed_launch_equals_per_shape_launches(bg, gpu, prec):
     # split-K off in both: the two launch forms have different workgroup slot
     # counts (occupancy per tile shape), so the planner may split K differently,
     # which changes the summation order
+    # (the 64x64 .. 128x128 shapes only: the 256x256 tile of packed FP16 keeps a
+    # launch of its own in both forms and could take a different share of the
+    # matrices in each)
+    no256 = sum(1 << t for t in range(9))
     for opts in ({"mixed_tiles": 1}, {"mixed_tiles": -1, "edge_tiles": -1}):
         vb.C.copy_(C0)
-        p = bg.Plan(M, N, K, prec=prec, split_k_min=-1, **opts)
+        p = bg.Plan(M, N, K, prec=prec, split_k_min=-1, tile_mask=no256, **opts)
         p.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
         torch.cuda.synchronize()
         outs.append((vb.C.clone(), p.kernels()))
