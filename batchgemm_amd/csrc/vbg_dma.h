@@ -322,6 +322,14 @@ struct DmaTile {
         }
       }
     };
+    // One whole-tile DMA instruction (chunk i of tile t into stage stg).
+    auto dma_chunk = [&](char* stg, int t, auto I) {
+      constexpr int i = decltype(I)::value;
+      const int k0 = kb + t * BK;
+      const uint32_t lbase = (uint32_t)(uintptr_t)stg + (uint32_t)(wave * 1024);
+      if constexpr (i < IA) vbg_dma16(rsA, vbg_u(lbase + 4096 * i), off(i), vbg_u(k0 * ksA));
+      else vbg_dma16(rsB, vbg_u(lbase + 4096 * i), off(i), vbg_u(k0 * ksB));
+    };
     // Wait for tile t (n tiles issued after it), and when it crosses the end
     // of the k range zero what this lane's DMAs placed at k >= end.
     auto arrive = [&](char* stg, int t, int n) {
@@ -361,6 +369,32 @@ struct DmaTile {
 #pragma unroll
       for (int c = 0; c < TN; ++c) f.b[c] = read_op<PB>(stg + PLANE_A, wc * WTN + c * 32, s, lane);
     };
+    // operand w of a step: A fragment w (w < TM) or B fragment w - TM
+    auto read_one = [&](Frags& f, const char* stg, int s, auto W) {
+      constexpr int w = decltype(W)::value;
+      if constexpr (w < TM) f.a[w] = read_op<PA>(stg, wr * WTM + w * 32, s, lane);
+      else f.b[w - TM] = read_op<PB>(stg + PLANE_A, wc * WTN + (w - TM) * 32, s, lane);
+    };
+    // MFMA j of a step (row j / TN, column j % TN), then the extras of its
+    // slot: the E extra instructions (LDS reads, DMAs) of a step are spread
+    // over the slots after its MFMAs 1 .. TM*TN-1, in order, each pair fenced,
+    // so every MFMA's pipe time covers the issue of the work beside it.
+    auto interleave = [&](const Frags& f, auto E, auto&& extra) {
+      constexpr int NE = decltype(E)::value;
+      constexpr int SL = TM * TN - 1;
+      if constexpr (SL == 0) {
+        vbg_static_for<NE>([&](auto X) { extra(X); });
+      } else {
+        vbg_static_for<SL>([&](auto J) {
+          constexpr int j = decltype(J)::value + 1;
+          acc[j / TN][j % TN] = mfma(f.a[j / TN], f.b[j % TN], acc[j / TN][j % TN]);
+          __builtin_amdgcn_sched_barrier(0);
+          constexpr int lo = (decltype(J)::value * NE) / SL, hi = ((decltype(J)::value + 1) * NE) / SL;
+          vbg_static_for<hi - lo>([&](auto X) { extra(std::integral_constant<int, lo + decltype(X)::value>{}); });
+          __builtin_amdgcn_sched_barrier(0);
+        });
+      }
+    };
     // MFMAs of one step; the first `from` were issued earlier
     auto mma = [&](const Frags& f, int from) {
 #pragma unroll
@@ -395,6 +429,7 @@ struct DmaTile {
     // before the first use of a transposing read's result, so reads issued
     // just before that use would stall it.)
     Frags f0, f1;
+    constexpr int NR = TM + TN;  // operand reads of a step
     auto body = [&](auto I, auto FAST, int t) {
       constexpr int i = decltype(I)::value;
       constexpr bool fast = decltype(FAST)::value;
@@ -405,10 +440,26 @@ struct DmaTile {
         Frags& nxt = even ? f1 : f0;
         acc[0][0] = mfma(cur.a[0], cur.b[0], acc[0][0]);
         __builtin_amdgcn_sched_barrier(0);
-        if constexpr (s + 1 < KS) read(nxt, stage(i), s + 1);
-        else if (fast || t + 1 < nk) next(I, FAST, t, nxt);
-        __builtin_amdgcn_sched_barrier(0);
-        mma(cur, 1);
+        if constexpr (s + 1 < KS) {
+          // the next step's reads spread over this step's MFMAs
+          interleave(cur, std::integral_constant<int, NR>{},
+                     [&](auto W) { read_one(nxt, stage(i), s + 1, W); });
+        } else if constexpr (fast) {
+          // steady state: tile t + 1 is whole and tile t + S a whole tile to
+          // fetch; after the barrier, the next tile's first reads, then the
+          // refill of this stage, spread over the MFMAs
+          vbg_wait_vm_barrier<(S - 2) * NI>();
+          __builtin_amdgcn_sched_barrier(0);
+          interleave(cur, std::integral_constant<int, NR + NI>{}, [&](auto X) {
+            constexpr int x = decltype(X)::value;
+            if constexpr (x < NR) read_one(nxt, stage((i + 1) % S), 0, std::integral_constant<int, x>{});
+            else dma_chunk(stage(i), t + S, std::integral_constant<int, x - NR>{});
+          });
+        } else {
+          if (t + 1 < nk) next(I, FAST, t, nxt);
+          __builtin_amdgcn_sched_barrier(0);
+          mma(cur, 1);
+        }
         __builtin_amdgcn_sched_barrier(0);
       });
     };
@@ -420,11 +471,12 @@ struct DmaTile {
       read(f0, stage(0), 0);
     }
     // Steady state (unrolled over the ring): every tile of the group has a
-    // successor that is a whole tile with S - 2 tiles issued after it, so the
-    // group runs without branches around its waits.
+    // successor that is a whole tile with S - 2 tiles issued after it, and
+    // the tile it refetches (t + S) is whole, so the group runs without
+    // branches around its waits and DMAs.
     const int nwhole = (ke - kb) / BK;
     int t = 0;
-    for (; t + S + S - 2 < nwhole; t += S)
+    for (; t + 2 * S - 1 < nwhole; t += S)
       vbg_static_for<S>([&](auto J) { body(J, std::true_type{}, t + decltype(J)::value); });
     for (; t + S <= nk; t += S)
       vbg_static_for<S>([&](auto J) { body(J, std::false_type{}, t + decltype(J)::value); });
