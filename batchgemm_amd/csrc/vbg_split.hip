@@ -488,233 +488,6 @@ vbg_split_kernel(vbg_args a) {
   vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
 }
 
-// ------------------------------------------------------- LDS-DMA packed tiles
-// Packed operands (k-contiguous rows of Kp, already quantised) streamed into
-// LDS by the buffer unit itself (buffer_load_dwordx4 ... lds: lane l's 16
-// bytes land at the instruction's LDS base + 16 l), through a ring of
-// DMA_STAGES stage buffers, so several k-tiles are in flight without holding
-// any of them in VGPRs: while the MFMAs consume tile t, tiles t+1 .. t+S-2
-// are landing and tile t+S-1 is issued.  The packed-operand loop of SplitTile
-// holds one k-tile in registers (lead time one k-tile), which the profiles
-// show waiting on its loads most of the time.
-//   Each stage buffer is its own __shared__ array, so the compiler's
-// load-counter tracking sees which DMA wrote which buffer and waits only for
-// that one before reading it; one barrier per k-tile, entered after this
-// wave's DMAs of the tile have landed (vmcnt) and its reads of the previous
-// stage have returned (lgkmcnt), makes every wave's part of the tile visible
-// and frees the previous stage for the next DMA.
-//   Stage layout: per plane [mn][k] rows of PITCH = BK*EB = 32 or 64 bytes in
-// 16-byte units, unit u of row r at position u ^ ((r / RPB) % NU) (RPB =
-// 256 / PITCH rows share a 256-byte bank line): the MFMA operand reads of 16
-// consecutive rows hit 16 distinct 16-byte bank groups.  A DMA instruction
-// fills 1 KB = 64 consecutive positions; each lane fetches the global unit that
-// belongs at its position.
-#ifndef VBG_DMA_STAGES
-#define VBG_DMA_STAGES 4
-#endif
-// Packed FP16 / FP8 GEMM kernels: 1 = the shared LDS-DMA tiles (vbg_dma.h,
-// with the 256x256 shape), 2 = the LDS-DMA tiles below, 0 = register staging.
-#ifndef VBG_PK_DMA
-#define VBG_PK_DMA 1
-#endif
-
-template <int MODE>
-struct DmaBK;  // k-tile depth: 64-byte LDS rows
-template <>
-struct DmaBK<MODE_FP16> {
-  static constexpr int value = 32;
-};
-template <>
-struct DmaBK<MODE_FP8> {
-  static constexpr int value = 64;
-};
-
-template <int BM, int BN, int MODE>
-struct DmaPkTile {
-  static_assert(MODE == MODE_FP16 || MODE == MODE_FP8, "packed FP16 / FP8 operands");
-  static constexpr int MODE_ID = MODE;
-  static constexpr int WAVES = 2;
-  static constexpr int S = VBG_DMA_STAGES;
-  static_assert(S == 3 || S == 4, "stage ring of 3 or 4 buffers");
-  static constexpr int EB = Mode<MODE>::EB, PLANES = Mode<MODE>::PLANES;
-  static constexpr int BK = DmaBK<MODE>::value;
-  static constexpr int PITCH = BK * EB;  // bytes per LDS row
-  static constexpr int NU = PITCH / 16;  // 16-byte units per row
-  static constexpr int RPB = 256 / PITCH;
-  static constexpr int PLANE_A = BM * PITCH, PLANE_B = BN * PITCH;
-  static constexpr int STAGE = PLANES * (PLANE_A + PLANE_B);
-  static constexpr int CHUNKS = STAGE / 1024;  // 1 KB DMA instructions per stage
-  static constexpr int NI = CHUNKS / 4;        // per wave
-  static_assert(STAGE % 4096 == 0 && (PLANES * PLANE_A) % 1024 == 0, "whole 1 KB chunks per wave");
-  static constexpr int WTM = BM / 2, WTN = BN / 2;
-  static constexpr int TM = WTM / 32, TN = WTN / 32;
-  static_assert(TM >= 1 && TN >= 1, "tile needs >= 64x64");
-
-  __device__ __forceinline__ static int swz(int row, int byte) {
-    return row * PITCH + ((((byte >> 4) ^ (row / RPB)) & (NU - 1)) << 4) + (byte & 15);
-  }
-
-  __device__ static void run(const vbg_args& a, const vbg_item& it, char* s0, char* s1, char* s2, char* s3) {
-    char* const st[4] = {s0, s1, s2, s3};
-    const vbg_mat_desc d = vbg_udesc(vbg_cload(a.desc, it.mat));
-    const int m0 = it.tm * BM, n0 = it.tn * BN;
-    const KRange kr = vbg_k_range<BK>(d.K, d.ksplit, it.kslice);
-    const int kb = kr.kb;
-    const int nk = (kr.ke - kb + BK - 1) / BK;
-    float post = 1.0f;
-    if (MODE != MODE_BF16X3)
-      post = __builtin_bit_cast(float, vbg_u(__builtin_bit_cast(int, vbg_cload(a.scales, 4 * it.mat + 2))));
-    const int lane = threadIdx.x & 63, wave = vbg_u(threadIdx.x >> 6);
-    const int wr = wave & 1, wc = wave >> 1;
-    const int l32 = lane & 31, h = lane >> 5;
-
-    // DMA sources: this wave's NI consecutive 1 KB chunks of a stage (the A
-    // planes, then the B planes; a wave's chunks may straddle the two, so the
-    // operand is chosen per chunk, uniformly over the wave).  Rows past MN
-    // read row MN-1 (their outputs are never stored).
-    const int Kp = (d.K + VBG_PACK_KALIGN - 1) & ~(VBG_PACK_KALIGN - 1);
-    const uint32_t planeA = (uint32_t)d.M * (uint32_t)Kp * EB, planeB = (uint32_t)d.N * (uint32_t)Kp * EB;
-    const v4u32_t rsA = vbg_rsrc_words(vbg_uptr(a.pk + vbg_u64(vbg_cload(a.pk_off, 2 * it.mat))), planeA * PLANES);
-    const v4u32_t rsB = vbg_rsrc_words(vbg_uptr(a.pk + vbg_u64(vbg_cload(a.pk_off, 2 * it.mat + 1))), planeB * PLANES);
-    int voff[NI];
-#pragma unroll
-    for (int i = 0; i < NI; ++i) {
-      const bool opA = (wave * NI + i) * 1024 < PLANES * PLANE_A;  // wave-uniform
-      int o = (wave * NI + i) * 1024 + 16 * lane;                  // byte within the stage
-      if (!opA) o -= PLANES * PLANE_A;
-      const int planesz = opA ? PLANE_A : PLANE_B;
-      const int plane = o / planesz, op = o % planesz;
-      const int row = op / PITCH, pos = (op % PITCH) >> 4;
-      const int unit = (pos ^ (row / RPB)) & (NU - 1);
-      const int MN = opA ? d.M : d.N, mn0 = opA ? m0 : n0;
-      voff[i] = (int)(plane * (opA ? planeA : planeB)) + min(mn0 + row, MN - 1) * Kp * EB + 16 * unit;
-    }
-    auto dma = [&](char* stage, int kt) {
-      const int soff = vbg_u((kb + kt * BK) * EB);
-      const uint32_t base = (uint32_t)(uintptr_t)stage + (uint32_t)(wave * NI * 1024);
-#pragma unroll
-      for (int i = 0; i < NI; ++i) {
-        const bool opA = (wave * NI + i) * 1024 < PLANES * PLANE_A;
-        vbg_dma16(opA ? rsA : rsB, vbg_u(base + i * 1024), voff[i], soff);
-      }
-    };
-
-    v16f acc[TM][TN];
-#pragma unroll
-    for (int r = 0; r < TM; ++r)
-#pragma unroll
-      for (int c = 0; c < TN; ++c)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) acc[r][c][e] = 0.0f;
-
-    auto compute = [&](const char* As) {
-      const char* Bs = As + PLANES * PLANE_A;
-#pragma unroll
-      for (int s = 0; s < BK / 16; ++s) {
-        const int koff = (16 * s + 8 * h) * EB;  // MFMA s: k in [16s, 16s + 16), lane half h: 8 of them
-        if (MODE == MODE_FP8) {
-          long av[TM], bv[TN];
-#pragma unroll
-          for (int r = 0; r < TM; ++r) av[r] = *(const long*)(As + swz(wr * WTM + r * 32 + l32, koff));
-#pragma unroll
-          for (int c = 0; c < TN; ++c) bv[c] = *(const long*)(Bs + swz(wc * WTN + c * 32 + l32, koff));
-#pragma unroll
-          for (int r = 0; r < TM; ++r)
-#pragma unroll
-            for (int c = 0; c < TN; ++c)
-              acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(av[r], bv[c], acc[r][c], 0, 0, 0);
-        } else if (MODE == MODE_FP16) {
-          v8h av[TM], bv[TN];
-#pragma unroll
-          for (int r = 0; r < TM; ++r) av[r] = *(const v8h*)(As + swz(wr * WTM + r * 32 + l32, koff));
-#pragma unroll
-          for (int c = 0; c < TN; ++c) bv[c] = *(const v8h*)(Bs + swz(wc * WTN + c * 32 + l32, koff));
-#pragma unroll
-          for (int r = 0; r < TM; ++r)
-#pragma unroll
-            for (int c = 0; c < TN; ++c)
-              acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av[r], bv[c], acc[r][c], 0, 0, 0);
-        }
-      }
-    };
-
-    // Ring: tile t in stage t % S.  Before tile t's MFMAs, its DMAs (issued S-1
-    // tiles earlier) must have landed for every wave: each wave waits until at
-    // most the DMAs of the tiles after t that are already issued (up to S-2 of
-    // them) remain, then all meet at the barrier.
-    auto wait_tile = [&](int t) {
-      const int ahead = min(S - 2, nk - 1 - t);  // tiles issued after t (uniform)
-      if (ahead >= 2) vbg_wait_vm_barrier<2 * NI>();
-      else if (ahead == 1) vbg_wait_vm_barrier<NI>();
-      else vbg_wait_vm_barrier<0>();
-    };
-    auto step = [&](auto I, int t) {
-      constexpr int i = decltype(I)::value;
-      wait_tile(t);
-      if (t + S - 1 < nk) dma(st[(i + S - 1) % S], t + S - 1);
-      compute(st[i]);
-    };
-#pragma unroll
-    for (int j = 0; j < S - 1; ++j)
-      if (j < nk) dma(st[j], j);
-    int t = 0;
-    for (; t + S <= nk; t += S) {
-      step(std::integral_constant<int, 0>{}, t);
-      step(std::integral_constant<int, 1>{}, t + 1);
-      step(std::integral_constant<int, 2>{}, t + 2);
-      if constexpr (S == 4) step(std::integral_constant<int, 3>{}, t + 3);
-    }
-    if (t < nk) step(std::integral_constant<int, 0>{}, t);
-    if (t + 1 < nk) step(std::integral_constant<int, 1>{}, t + 1);
-    if (t + 2 < nk) step(std::integral_constant<int, 2>{}, t + 2);
-    // no stage may be refilled by the next item before every wave is done
-    // reading this item's last tile
-    vbg_wait_vm_barrier<0>();
-
-    constexpr int NSEG = TM * TN * 4;
-    const OutTile o = vbg_out(a, d, it, post);
-    auto seg = [&](int sg, int& row, int& col, v4f& v) {
-      const int r = sg / (TN * 4), c = (sg / 4) % TN, g = sg % 4;
-      row = m0 + wr * WTM + r * 32 + 8 * g + 4 * h;
-      col = n0 + wc * WTN + c * 32 + l32;
-      v = v4f{acc[r][c][4 * g], acc[r][c][4 * g + 1], acc[r][c][4 * g + 2], acc[r][c][4 * g + 3]};
-    };
-    vbg_epilogue<NSEG>(o, d.M, d.N, seg);
-  }
-};
-
-// Mixed launch of the DMA tiles: the 64x64 .. 128x128 shapes share the ring,
-// sized for the largest.
-template <int MODE>
-struct DmaPkMixed {
-  using Sq128 = DmaPkTile<128, 128, MODE>;
-  using Tall = DmaPkTile<128, 64, MODE>;
-  using Wide = DmaPkTile<64, 128, MODE>;
-  using Sq64 = DmaPkTile<64, 64, MODE>;
-  static constexpr int MODE_ID = MODE;
-  static constexpr int WAVES = 2;
-  static constexpr int S = Sq128::S;
-  static constexpr int STAGE = Sq128::STAGE;
-  __device__ static void run(const vbg_args& a, const vbg_item& it, char* s0, char* s1, char* s2, char* s3) {
-    switch (it.tile) {
-      case T128: Sq128::run(a, it, s0, s1, s2, s3); break;
-      case T128x64: Tall::run(a, it, s0, s1, s2, s3); break;
-      case T64x128: Wide::run(a, it, s0, s1, s2, s3); break;
-      default: Sq64::run(a, it, s0, s1, s2, s3); break;
-    }
-  }
-};
-
-template <class Tile>
-__global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(Tile::WAVES)))
-vbg_dma_kernel(vbg_args a) {
-  __shared__ __attribute__((aligned(1024))) char s0[Tile::STAGE];
-  __shared__ __attribute__((aligned(1024))) char s1[Tile::STAGE];
-  __shared__ __attribute__((aligned(1024))) char s2[Tile::STAGE];
-  __shared__ __attribute__((aligned(1024))) char s3[Tile::S == 4 ? Tile::STAGE : 1024];
-  vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, s0, s1, s2, s3); });
-}
-
 // ---------------------------------------------------------------- operand pack
 // Operand pack pre-pass: job j quantises rows [row0, row0 + 64) x k [k0, k0 +
 // VBG_PACK_K) of one operand into the packed layout PkLoader reads -- rows of
@@ -953,38 +726,19 @@ static int dispatch_tile(int tile, bool ta, bool tb, const vbg_args* a, int grid
   }
 }
 
-template <class Tile>
-static int run_or_occ_dma(const vbg_args* a, int grid, hipStream_t s, int* occ) {
-  if (occ) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, vbg_dma_kernel<Tile>, VBG_THREADS, 0) != hipSuccess)
-      nb = 1;
-    *occ = nb > 0 ? nb : 1;
-    return 0;
-  }
-  hipLaunchKernelGGL(vbg_dma_kernel<Tile>, dim3(grid), dim3(VBG_THREADS), 0, s, *a);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
-}
-
-// Packed FP16 / FP8 run on the LDS-DMA tiles; packed bf16x3 (two planes, so
-// its 80 KB per workgroup holds four stages only at 16 k each) keeps the
-// register-staged loop.  Unbalanced B=2048 batch, GEMM kernel ms, register
-// staging / DMA ring: FP16 2.19 / 2.19, FP8 1.85 / 1.61, bf16x3 3.53 / 4.82.
+// Packed FP16 / FP8 run on the shared LDS-DMA tiles (vbg_dma.h): FP16 on the
+// 32x32x16 MFMA with the 256x256 shape for the big matrices, FP8 on the
+// 32x32x64 f8f6f4 MFMA.  Packed bf16x3 (two planes per operand) keeps the
+// register-staged loop.  Unbalanced B=2048 batch, GEMM kernel ms: the earlier
+// 128x128 DMA ring (64-byte rows, 4 stages, one wait per tile) FP16 2.21 /
+// FP8 1.60; these tiles FP16 1.94 + 0.08 / FP8 1.59; register staging FP16
+// 2.19, FP8 1.85.
+#ifndef VBG_PK_DMA
+#define VBG_PK_DMA 1
+#endif
 template <int MODE>
 static int dispatch_tile_pk(int tile, const vbg_args* a, int grid, hipStream_t s, int* occ) {
-  if constexpr (VBG_PK_DMA == 2 && MODE != MODE_BF16X3) {
-    switch (tile) {
-      case T64: return run_or_occ_dma<DmaPkTile<64, 64, MODE>>(a, grid, s, occ);
-      case T128x64: return run_or_occ_dma<DmaPkTile<128, 64, MODE>>(a, grid, s, occ);
-      case T64x128: return run_or_occ_dma<DmaPkTile<64, 128, MODE>>(a, grid, s, occ);
-      case T128: return run_or_occ_dma<DmaPkTile<128, 128, MODE>>(a, grid, s, occ);
-      case T_MIXED: return run_or_occ_dma<DmaPkMixed<MODE>>(a, grid, s, occ);
-      default: return -1;
-    }
-  }
-  if constexpr (VBG_PK_DMA == 1 && MODE != MODE_BF16X3) {
-    // the shared LDS-DMA tiles (vbg_dma.h): FP16 on the 32x32x16 MFMA, FP8 on
-    // the 32x32x64 f8f6f4 MFMA
+  if constexpr (VBG_PK_DMA && MODE != MODE_BF16X3) {
     constexpr int DM = MODE == MODE_FP16 ? DM_PK_FP16 : DM_PK_FP8;
     switch (tile) {
       case T64: return vbg_run_dmatile<DmaTile<64, 64, true, false, DM>>(a, grid, s, occ);
