@@ -72,13 +72,17 @@ def kernel_name(bg, prec, kind, tile):
     if kind == 6:
         return f"vbg_pack_kernel[{prec}]"
     tag = {3: f"[{prec}]", 7: f"[{prec},packed]"}.get(kind, "")
+    # packed FP16 / FP8 and the BF16 family run the shared LDS-DMA tiles
+    dma = kind == 4 or (kind == 7 and prec in ("fp16", "fp8"))
     if tile == bg._lib.TILE_MIXED:  # one launch over the 64x64 .. 128x128 shapes
+        if dma:
+            return f"DmaTileMixed{tag}"
         return {1: "Fp32Mixed", 3: f"SplitMixed{tag}", 7: f"SplitMixed{tag}"}.get(
             kind, f"{KERNEL_TEMPLATE[kind]}<mixed>")
     bm, bn = bg.TILE_SHAPES[tile]
     if kind == 1 and bm == bn and bm <= 32:
         return f"Fp32SmallTile<{bm}>"
-    return f"{KERNEL_TEMPLATE[kind]}<{bm},{bn}>" + tag
+    return f"{'DmaTile' if dma else KERNEL_TEMPLATE[kind]}<{bm},{bn}>" + tag
 
 
 TRAFFIC_FILE = os.path.join("profiles", "traffic_unbalanced_b2048.txt")
