@@ -510,9 +510,11 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
   // launch is short: a small fraction of the dominant one, or no longer than
   // a launch's fixed cost (tiny batches, where every launch is latency).
   // Measured: bf16x3 on the unbalanced B=2048 batch runs in one launch
-  // instead of four (220 -> 223 TFLOP/s).
+  // instead of four (220 -> 223 TFLOP/s); with "small" at 10% of the
+  // dominant launch's work rather than 5%, its 256x128 launch also takes the
+  // 128x128 launch holding 5.5% of the work (3.62-3.66 -> 3.56-3.60 ms).
   if (o.policy == VBG_POLICY_AUTO && o.consolidate >= 0 && fam != FAM_SEQ && batch > 0) {
-    const double kLaunchFixed = 8.0, kAbsorbFrac = 0.05;
+    const double kLaunchFixed = 8.0, kAbsorbFrac = 0.10;
     double tot[NTILES] = {}, longest[NTILES] = {};
     for (int b = 0; b < batch; ++b) {
       if (M[b] == 0 || N[b] == 0) continue;
