@@ -257,6 +257,11 @@ int vbg_ref_plan(int batch, const int* M, const int* N, const int* K, int wavefr
  * A and C/dC/dA are stacked over groups (sum of M_g rows, group g starts at
  * row offset sum_{h<g} M_h); B / dB are [G, K, N].  dtype codes: 0 = fp32,
  * 1 = bf16 for the C / dC / dA / dB buffers.  Groups with M_g = 0 write zero dB.
+ * A plan owns one side stream: the kernels of a pass's few-row groups run on
+ * it beside the GEMM of the large groups, forked from and joined back to
+ * `stream` by events, so a call is still ordered on `stream` (and can be
+ * captured into a HIP graph).  Calls on one plan must not overlap in time:
+ * use one plan per stream.
  */
 typedef struct vbg_grouped_plan vbg_grouped_plan;
 int vbg_grouped_plan_create(vbg_grouped_plan** plan, int G, const int* M_host, int K, int N,

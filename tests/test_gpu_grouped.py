@@ -249,3 +249,41 @@ def test_grouped_small_groups_ragged_streams(bg, gpu):
         assert rel(C32[s:e], C[s:e]) <= 1e-5, ("C", g, m_sizes[g])
         assert rel(dA[s:e], rA[s:e]) <= 1e-5, ("dA", g, m_sizes[g])
         assert rel(dB[g], rB[g]) <= 1e-5, ("dB", g, m_sizes[g])
+
+
+def test_grouped_graph_replays_match_eager(bg, gpu):
+    """The differentiation workload's passes -- few-row groups on the plan's
+    side stream beside the large group's GEMM, forked from and joined to the
+    caller's stream -- captured into a HIP graph: every replay equals the
+    eager run bit for bit, with eager work and synchronisations between
+    replays."""
+    import torch
+    from batchgemm_amd import workloads as w
+    from batchgemm_amd.grouped import GroupedPlan
+    say("[grouped] graph replays, config4 shapes")
+    m_sizes = w.zipf_groups(128, 32768, 1.2, seed=0)
+    K, N = 2048, 1408
+    A, B, dC = make(m_sizes, K, N, seed=3)
+    p = GroupedPlan(m_sizes, K, N)
+    C_e = p.forward(A, B)
+    dA_e, dB_e = p.backward(A, B, dC)
+    torch.cuda.synchronize()
+    Cg = torch.empty_like(C_e)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        p.forward(A, B, out=Cg)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        p.forward(A, B, out=Cg)
+        dAg, dBg = p.backward(A, B, dC)
+    for rep in range(3):
+        for t in (Cg, dAg, dBg):
+            t.zero_()  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for name, got, ref in (("C", Cg, C_e), ("dA", dAg, dA_e), ("dB", dBg, dB_e)):
+            assert torch.equal(got.view(torch.int16), ref.view(torch.int16)), (name, rep)
