@@ -538,7 +538,7 @@ template <class Tile>
 __global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(Tile::WAVES)))
 vbg_dmatile_kernel(vbg_args a) {
   __shared__ __attribute__((aligned(1024))) char lds[Tile::LDS_BYTES];
-  vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
+  vbg_persistent_loop<2>(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
 }
 
 // Launch (or, with occ, report the workgroups per CU of) a DMA-tile kernel.

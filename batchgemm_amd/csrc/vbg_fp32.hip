@@ -415,11 +415,15 @@ struct min_waves<T, decltype((void)T::MIN_WAVES)> {
   static constexpr int value = T::MIN_WAVES;
 };
 
-template <class Tile>
+// LOOK: the queue walk of launches with few items per workgroup (look_all).
+// It is a separate instantiation because a walk chosen at run time changes
+// the code the compiler makes for the FP32 bodies (4% slower on large
+// launches that never take it).
+template <class Tile, bool LOOK>
 __global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(min_waves<Tile>::value)))
 vbg_gemm_kernel(vbg_args a) {
   __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
-  vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
+  vbg_persistent_loop<LOOK ? 1 : 0>(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
 }
 
 // Mixed launch: one persistent grid over items of the 64x64 .. 128x128
@@ -457,14 +461,17 @@ struct Fp32Mixed {
 
 template <class Tile>
 static hipError_t launch_one(const vbg_args& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(vbg_gemm_kernel<Tile>, dim3(grid), dim3(VBG_THREADS), 0, s, a);
+  if (a.look_all)
+    hipLaunchKernelGGL((vbg_gemm_kernel<Tile, true>), dim3(grid), dim3(VBG_THREADS), 0, s, a);
+  else
+    hipLaunchKernelGGL((vbg_gemm_kernel<Tile, false>), dim3(grid), dim3(VBG_THREADS), 0, s, a);
   return hipGetLastError();
 }
 
 template <class Tile>
 static int occupancy_one() {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, vbg_gemm_kernel<Tile>, VBG_THREADS, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, vbg_gemm_kernel<Tile, false>, VBG_THREADS, 0) !=
       hipSuccess)
     return 1;
   return nb > 0 ? nb : 1;

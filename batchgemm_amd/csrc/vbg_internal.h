@@ -56,6 +56,7 @@ struct vbg_args {
   const int64_t* pk_off;  // [2*batch] byte offsets of packed A and B in pk
   float alpha, beta;
   int nq;
+  int look_all;           // 1: a workgroup whose queue is empty reads every queue's head at once
   int trans_a, trans_b;   // 0 = N, 1 = T
   int prec;               // vbg_prec
 };

@@ -157,6 +157,9 @@ int issue_run(const HostPlan& hp, const DeviceView& v, float alpha, const float*
     a.alpha = alpha;
     a.beta = beta;
     a.nq = hp.nq;
+    // a launch with at most two items per workgroup spends a large share of
+    // its time in the final walk over the queues: look at all of them at once
+    a.look_all = (int64_t)L.items.size() <= 2 * (int64_t)L.grid ? 1 : 0;
     a.trans_a = hp.ta;
     a.trans_b = hp.tb;
     a.prec = hp.prec;

@@ -485,7 +485,7 @@ template <class Tile>
 __global__ void __launch_bounds__(VBG_THREADS) __attribute__((amdgpu_waves_per_eu(Tile::WAVES)))
 vbg_split_kernel(vbg_args a) {
   __shared__ __attribute__((aligned(16))) float lds[Tile::LDS_FLOATS];
-  vbg_persistent_loop(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
+  vbg_persistent_loop<2>(a, [&](const vbg_item& it) { Tile::run(a, it, lds); });
 }
 
 // ---------------------------------------------------------------- operand pack

@@ -253,7 +253,10 @@ struct TileLoader {
 // workgroup ids are dealt round-robin to the XCDs) and moves on to the next
 // queues once its own is drained.  The last workgroup to finish resets the
 // counters, so consecutive launches on one plan need no memset.
-template <class Body>
+// WALK picks how a workgroup moves past drained queues: 0 claims queue by
+// queue, 1 looks at every queue at once (pop_look below), 2 takes 1 when the
+// launch's look_all is set and 0 otherwise.
+template <int WALK, class Body>
 __device__ __forceinline__ void vbg_persistent_loop(const vbg_args& a, Body&& body) {
   __shared__ int s_item;
   const int nq = a.nq;
@@ -272,13 +275,66 @@ __device__ __forceinline__ void vbg_persistent_loop(const vbg_args& a, Body&& bo
     }
     return -1;
   };
+  // wave 0, launches of few items per workgroup: claim from the own queue
+  // while it has items, then read every queue's head at once (a lane per
+  // queue) and claim from the first queue at or after q that still has
+  // items, so the last pop of a workgroup costs one read instead of a
+  // claim per queue.  Heads only grow during a launch, so a queue read as
+  // drained is drained.  Every lane follows the same path and values.
+  auto pop_look = [&]() -> int {
+    const int lane = threadIdx.x;
+    for (;;) {
+      if (tried == 0) {
+        int v = -1;
+        if (lane == 0) {
+          const int lo = a.q_begin[q], hi = a.q_begin[q + 1];
+          if (lo < hi) {
+            const unsigned h = atomicAdd(&a.counters[q], 1u);
+            if ((int)h < hi - lo) v = lo + (int)h;
+          }
+        }
+        v = __shfl(v, 0);
+        if (v >= 0) return v;
+        tried = 1;
+      }
+      int left = 0;
+      if (lane < nq) {
+        const int cnt = a.q_begin[lane + 1] - a.q_begin[lane];
+        const unsigned h =
+            __hip_atomic_load(&a.counters[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        left = (int)h < cnt;
+      }
+      unsigned long long m = __ballot(left);
+      if (m == 0) return -1;
+      // rotate so that bit 0 is queue q, then take the lowest set bit (only
+      // bits 0 .. nq-1 are set, nq <= 64)
+      if (q > 0) m = ((m >> q) | (m << (nq - q))) & (nq == 64 ? ~0ull : (1ull << nq) - 1);
+      const int qq = q + __builtin_ctzll(m);
+      q = qq >= nq ? qq - nq : qq;
+      int v = -1;
+      if (lane == 0) {
+        const int lo = a.q_begin[q], hi = a.q_begin[q + 1];
+        const unsigned h = atomicAdd(&a.counters[q], 1u);
+        if ((int)h < hi - lo) v = lo + (int)h;
+      }
+      v = __shfl(v, 0);
+      if (v >= 0) return v;
+    }
+  };
   // the item index is wave-uniform; saying so (readfirstlane) lets the
   // compiler keep the item, its descriptor, pointers and k offsets in scalar
   // registers.  (Claiming the next item while the current one runs measured
   // no better: FP32 -1.5% on the batching workload, the split precisions
   // within +-1%.)
   for (;;) {
-    if (threadIdx.x == 0) s_item = pop();
+    if (WALK == 1 || (WALK == 2 && a.look_all)) {
+      if (threadIdx.x < 64) {
+        const int v = pop_look();
+        if (threadIdx.x == 0) s_item = v;
+      }
+    } else if (threadIdx.x == 0) {
+      s_item = pop();
+    }
     __syncthreads();
     const int it = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
