@@ -250,19 +250,30 @@ def test_scaled_paths_tiny_magnitudes(bg, orc, gpu, prec):
         assert orc.rel_fro(got, exact) <= FP64_BOUND[prec], b
 
 
+@pytest.mark.parametrize("pad", [0, 3])
 @pytest.mark.parametrize("trans", ["NN", "TN", "NT", "TT"])
-def test_bf16x3_256x128_tile(bg, orc, gpu, trans):
+def test_bf16x3_256x128_tile(bg, orc, gpu, trans, pad):
     """The bf16x3 family's 256x128 tile (16-deep k-tiles, 128 x 64 per wave)
-    on ragged shapes -- rows past M in a partial 256-row tile, K not a
-    multiple of 16, K = 0 -- against the emulation and FP64, and bit-equal to
-    the 128x128 tile (the k grouping into MFMAs is the same)."""
+    on ragged shapes -- rows past M in a partial 256-row tile, tiles with 1,
+    2 or 3 rows inside the operand and M < 4 (the m-contiguous loader's
+    element-by-element path), K not a multiple of 16, K = 0, leading
+    dimensions padded with 1e30 -- against the emulation and FP64, and
+    bit-equal to the 128x128 tile (the k grouping into MFMAs is the same)."""
     import torch
     from batchgemm_amd.workloads import VBatch
-    say(f"[bf16x3 256x128] trans={trans}")
-    M = [256, 300, 17, 130, 513, 64]
-    N = [128, 33, 140, 129, 250, 1]
-    K = [64, 200, 9, 257, 35, 0]
-    vb = VBatch(M, N, K, trans[0], trans[1]).fill("normal", seed=19, outlier_frac=0.01, c_dist="uniform")
+    say(f"[bf16x3 256x128] trans={trans} pad={pad}")
+    M = [256, 300, 17, 130, 513, 64, 258, 3, 1, 259, 20]
+    N = [128, 33, 140, 129, 250, 1, 40, 50, 3, 2, 130]
+    K = [64, 200, 9, 257, 35, 0, 48, 17, 33, 16, 1]
+    vb = VBatch(M, N, K, trans[0], trans[1], pad_ab=pad).fill("normal", seed=19, outlier_frac=0.01,
+                                                                c_dist="uniform")
+    if pad:
+        # padding holds huge values: a load that used it would spoil the product
+        for pool, off, rows, cols, ld in ((vb.A, vb.offA, vb.a_rows, vb.a_cols, vb.lda),
+                                          (vb.B, vb.offB, vb.b_rows, vb.b_cols, vb.ldb)):
+            for b in range(vb.batch):
+                v = pool[int(off[b]): int(off[b]) + int(ld[b]) * int(cols[b])].view(int(cols[b]), int(ld[b]))
+                v[:, int(rows[b]):] = 1e30
     C0 = vb.C.clone()
     out = {}
     for tile in ("256x128", "128x128"):
@@ -283,7 +294,11 @@ def test_bf16x3_256x128_tile(bg, orc, gpu, trans):
         emul = 1.5 * emulate(orc, "bf16x3", A, B) + 0.5 * c0
         exact = 1.5 * (A.astype(np.float64) @ B.astype(np.float64)) + 0.5 * c0
         assert orc.rel_fro(got, emul) <= EMUL_BOUND["bf16x3"], b
-        assert orc.rel_fro(got, exact) <= FP64_BOUND["bf16x3"], b
+        # a small matrix (K = 9, 1% outliers) can put the bf16x3 algorithm
+        # itself past the FP64 bound: then the kernel stays within the
+        # emulation bound of the algorithm's own error
+        bound = max(FP64_BOUND["bf16x3"], orc.rel_fro(emul, exact) + EMUL_BOUND["bf16x3"])
+        assert orc.rel_fro(got, exact) <= bound, b
 
 
 def test_256x128_tile_refused_outside_bf16x3(bg, gpu):
