@@ -274,7 +274,28 @@ def extras(bg, w, vb, M, N, K, alpha, beta, flops, fp32_value, args):
     out["grouped_bf16"] = bench_config4(args.steps, args.warmup, args.seed)
     f_ms, b_ms = out["grouped_bf16"]["fwd_ms"], out["grouped_bf16"]["bwd_ms"]
     log(f"[bench] grouped fwd {f_ms:.3f} ms bwd {b_ms:.3f} ms")
+    out["small_batch"] = small_batch_latency(bg, w, args)
     return out
+
+
+def small_batch_latency(bg, w, args):
+    """The reference's small random batch (B = 64, M, N, K ~ U{16..256},
+    alpha 1, beta 0, U[-1, 1] inputs): a latency case, one launch of a few
+    hundred items.  Microseconds per batch back to back and with a host
+    synchronisation after each."""
+    M, N, K = w.shapes_uniform(64, 16, 256, seed=0)
+    vb = w.VBatch(M, N, K).fill("uniform", seed=0)
+    flops = 2.0 * float(np.sum(M.astype(np.float64) * N * K))
+    res = {"batch": "B=64, M,N,K ~ U{16..256}, seed 0", "gflop": round(flops / 1e9, 4)}
+    for prec in ("fp32", "bf16x3"):
+        p = bg.Plan(M, N, K, prec=prec)
+        run = lambda: p.run(vb.dA, vb.dB, vb.dC, 1.0, 0.0)
+        us = 1e3 * time_launches(run, 200, 20)
+        us_sync = 1e3 * time_synced(run, 100, 10)
+        res[prec] = {"us": round(us, 1), "sync_us": round(us_sync, 1),
+                     "gflops": round(flops / (us * 1e-6) / 1e9, 1)}
+        log(f"[bench] small batch {prec}: {us:.1f} us back to back, {us_sync:.1f} us synchronised")
+    return res
 
 
 def traffic_probe(bg, vb, M, N, K, alpha, beta, args):
