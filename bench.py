@@ -401,7 +401,9 @@ def main():
     iflops = tplan.item_flops()
     rows = []
     for (kind, tile, first, cnt, grid), t in zip(kt, kms):
-        f = float(iflops[first:first + cnt].sum()) if kind in (1, 2, 3, 4) else 0.0
+        # GEMM launches (FP32, sequential, split, BF16, split on packed
+        # operands) carry flops; amax, split-K reduction and pack passes none
+        f = float(iflops[first:first + cnt].sum()) if kind in (1, 2, 3, 4, 7) else 0.0
         rows.append({"kernel": kernel_name(bg, args.prec, int(kind), int(tile)), "items": int(cnt),
                      "grid": int(grid), "ms": round(float(t), 4), "gflop": round(f / 1e9, 3),
                      "tflops": round(f / (t * 1e-3) / 1e12, 1) if f and t > 0 else None})
