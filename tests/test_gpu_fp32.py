@@ -167,3 +167,32 @@ def test_kernel_timing_plan(bg, orc, gpu):
     with pytest.raises(VbgError):
         with torch.cuda.graph(g, stream=s):
             plan.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5, stream=s)
+
+
+@pytest.mark.parametrize("prec,tile", [("fp32", "128x128"), ("fp32", "64x64"), ("bf16x3", "128x128")])
+def test_result_independent_of_queue_walk(bg, gpu, prec, tile):
+    """The same matrices give the same bits in a launch of at most two items
+    per workgroup, where a workgroup whose queue is drained reads every queue
+    head at once, and at the head of a large batch, where it claims queue by
+    queue: an item's result does not depend on the workgroup that runs it."""
+    import torch
+    from batchgemm_amd import workloads as w
+    say(f"[walk] prec={prec} tile={tile}")
+    M0, N0, K0 = w.shapes_uniform(24, 16, 300, seed=7)
+    M1, N1, K1 = w.shapes_uniform(1500, 64, 300, seed=8)
+    M, N, K = (np.concatenate([a, b]) for a, b in ((M0, M1), (N0, N1), (K0, K1)))
+    vb = w.VBatch(M, N, K).fill("normal", seed=3, c_dist="uniform")
+    C0 = vb.C.clone()
+    opts = dict(prec=prec, policy="fixed", fixed_tile=tile, mixed_tiles=-1, split_k_min=-1)
+    small, big = bg.Plan(M0, N0, K0, **opts), bg.Plan(M, N, K, **opts)
+    # rows (kind, tile, first item, items, grid); FP32 adds edge-strip launches
+    ks, kb = small.kernels(), big.kernels()
+    assert (ks[:, 3] <= 2 * ks[:, 4]).all() and kb[0, 3] > 2 * kb[0, 4], (ks, kb)
+    small.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
+    torch.cuda.synchronize()
+    got_small = [vb.host_flat("C", b) for b in range(len(M0))]
+    vb.C.copy_(C0)
+    big.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
+    torch.cuda.synchronize()
+    for b in range(len(M0)):
+        assert np.array_equal(got_small[b].view(np.int32), vb.host_flat("C", b).view(np.int32)), b
