@@ -345,6 +345,9 @@ def main():
                     help="--traffic-probe only: the unbalanced B=2048 batch (default), a "
                          "uniform 64 x 2048^3 batch on 128x128 tiles, or the first batch of the "
                          "batching workload (B=7007, M,N,K ~ log-U[8,1024], alpha 1, beta 0)")
+    ap.add_argument("--small-batch", action="store_true",
+                    help="only the small random batch (B = 64, M,N,K ~ U{16..256}): FP32 and "
+                         "bf16x3 microseconds per batch, as one JSON object (for kernel traces)")
     args = ap.parse_args()
     precs = args.prec.split(",")
     for pr in precs:
@@ -363,6 +366,9 @@ def main():
     import batchgemm_amd as bg
     from batchgemm_amd import workloads as w
 
+    if args.small_batch:
+        print(json.dumps(small_batch_latency(bg, w, args)))
+        return
     alpha, beta = 1.5, 0.5
     if args.traffic_probe and args.workload == "uniform":
         M = N = K = np.full(64, 2048, np.int32)
