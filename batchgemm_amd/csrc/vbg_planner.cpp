@@ -142,7 +142,7 @@ bool tile_allowed(int prec, int tile, unsigned mask) {
   // (packed FP8's 256x256 body, two sets of 32-byte f8f6f4 operands beside 256
   // accumulators, does not fit the register file)
   const bool pk_dma = g_cm.packed && prec == VBG_PREC_FP16_SCALED;
-  if (tile == T256x128 && prec != VBG_PREC_BF16X3 && !bf16) return false;
+  if (tile == T256x128 && prec != VBG_PREC_BF16X3 && !bf16 && !pk_dma) return false;
   if (tile == T256 && !bf16 && !pk_dma) return false;
   if ((tile == T16x128 || tile == T128x16) && prec != VBG_PREC_FP32) return false;
   return true;
@@ -165,6 +165,11 @@ void choose_auto(const std::vector<Shape>& sh, int prec, const vbg_opts& o, int 
     for (int t = 0; t < NTILES; ++t) {
       if (!tile_allowed(prec, t, o.tile_mask)) continue;
       if (t == T128 && big_min && std::min(sh[b].M, sh[b].N) < big_min) continue;
+      // packed FP16 runs 256x128 rather than 256x256 (two workgroups per CU
+      // instead of one): unbalanced B=2048 batch 3.32 vs 3.43 ms, B=512 0.890
+      // vs 0.898 ms; 64 x 2048^3 2.584 vs 2.557-2.562 ms (256x256 stays a
+      // fixed-policy choice)
+      if (t == T256 && prec == VBG_PREC_FP16_SCALED) continue;
       const double c = matrix_cost(t, sh[b]);
       if (best < 0 || c < bc - 1e-9) {
         best = t;
@@ -456,8 +461,8 @@ int build_plan(HostPlan& hp, int batch, const int* M, const int* N, const int* K
           return VBG_ERR_INVALID_ARG;
         }
         if ((o.fixed_tile == T256x128 || o.fixed_tile == T256) && !tile_allowed(prec, o.fixed_tile, 0)) {
-          err = "the 256x128 tile is built for bf16x3 and BF16 operands, 256x256 for BF16 and packed "
-                "FP16 / FP8 operands only";
+          err = "the 256x128 tile is built for bf16x3, BF16 and packed FP16 operands, 256x256 for BF16 and "
+                "packed FP16 operands only";
           return VBG_ERR_NOT_SUPPORTED;
         }
         if ((o.fixed_tile == T16x128 || o.fixed_tile == T128x16) && prec != VBG_PREC_FP32) {

@@ -745,6 +745,11 @@ static int dispatch_tile_pk(int tile, const vbg_args* a, int grid, hipStream_t s
       case T128x64: return vbg_run_dmatile<DmaTile<128, 64, true, false, DM>>(a, grid, s, occ);
       case T64x128: return vbg_run_dmatile<DmaTile<64, 128, true, false, DM>>(a, grid, s, occ);
       case T128: return vbg_run_dmatile<DmaTile<128, 128, true, false, DM>>(a, grid, s, occ);
+      case T256x128:
+        // FP16 only: FP8's one-step k-tile needs an even ring, which at
+        // 256x128 leaves one workgroup per CU (measured slower)
+        if constexpr (DM == DM_PK_FP16) return vbg_run_dmatile<DmaTile<256, 128, true, false, DM>>(a, grid, s, occ);
+        return -1;
       case T256:
         if constexpr (DM == DM_PK_FP16) return vbg_run_dmatile<DmaTile<256, 256, true, false, DM>>(a, grid, s, occ);
         return -1;

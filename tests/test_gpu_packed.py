@@ -118,3 +118,34 @@ def test_packed_graph_replays(bg, gpu, prec):
         g.replay()
         torch.cuda.synchronize()
         assert torch.equal(vb2.C.view(torch.int32), E2.view(torch.int32)), ("rebound replay", rep)
+
+
+@pytest.mark.parametrize("trans", ["NN", "TN", "NT", "TT"])
+def test_packed_fp16_256x128_equals_in_loop(bg, gpu, trans):
+    """Packed FP16 on the 256x128 LDS-DMA tile (the AUTO choice for its big
+    matrices) gives the bits of FP16 quantised in the k-loop on 128x128
+    tiles: the same quantised operands, and the k grouping into
+    32x32x16 f16 MFMAs is the same for every tile shape and k-tile depth."""
+    import torch
+    from batchgemm_amd.workloads import VBatch
+    M, N, K = (np.asarray(x, np.int32) for x in SHAPES)
+    say(f"[packed] fp16 256x128 trans={trans}")
+    vb = VBatch(M, N, K, trans[0], trans[1], pad_ab=3)
+    common = dict(prec="fp16", policy="fixed", mixed_tiles=-1)
+    packed = bg.Plan(M, N, K, vb.lda, vb.ldb, vb.ldc, trans[0], trans[1], pack_operands=1,
+                     fixed_tile="256x128", **common)
+    in_loop = bg.Plan(M, N, K, vb.lda, vb.ldb, vb.ldc, trans[0], trans[1], pack_operands=-1,
+                      fixed_tile="128x128", **common)
+    assert (packed.kernels()[:, 1] == 6).any() and packed.packed_bytes > 0
+    for seed in (43, 44):
+        vb.fill("normal", seed=seed, outlier_frac=0.01, c_dist="uniform")
+        poison_padding(vb)
+        C0 = vb.C.clone()
+        out = []
+        for p in (packed, in_loop):
+            vb.C.copy_(C0)
+            p.run(vb.dA, vb.dB, vb.dC, 1.5, 0.5)
+            torch.cuda.synchronize()
+            out.append(vb.C.clone())
+        assert torch.isfinite(out[0]).all()
+        assert torch.equal(out[0].view(torch.int32), out[1].view(torch.int32)), seed

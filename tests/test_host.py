@@ -335,20 +335,26 @@ def test_package_dataset_is_the_reference_dataset(bg):
     assert open(path, "rb").read() == open(DATA, "rb").read()
 
 
-def test_256x128_tile_is_bf16x3_and_bf16_only(bg):
-    """AUTO plans use the 256x128 tile for bf16x3 and BF16 operands only
-    (unless the tile mask leaves it out); FIXED refuses it for other
-    precisions."""
+def test_256x128_tile_is_bf16x3_bf16_and_packed_fp16_only(bg):
+    """AUTO plans use the 256x128 tile for bf16x3, BF16 and packed FP16
+    operands only (unless the tile mask leaves it out), and packed FP16 never
+    the 256x256 one; FIXED refuses 256x128 for the other precisions and for
+    FP16 quantised in the k-loop."""
     from batchgemm_amd import workloads as w
     M, N, K = w.shapes_unbalanced(256, seed=4)
     all_but = sum(1 << t for t in range(6))
     for prec in ("fp32", "bf16x3", "fp16", "fp8"):
-        uses = (bg.Plan(M, N, K, prec=prec, host_only=True).tiles() == 6).any()
-        assert uses == (prec == "bf16x3"), prec
+        p = bg.Plan(M, N, K, prec=prec, host_only=True)
+        uses = (p.tiles() == 6).any()
+        assert uses == (prec in ("bf16x3", "fp16")), prec
+        assert not (p.tiles() == 9).any(), prec
         assert not (bg.Plan(M, N, K, prec=prec, host_only=True, tile_mask=all_but).tiles() == 6).any()
-    p = bg.Plan(M, N, K, prec="bf16x3", host_only=True, policy="fixed", fixed_tile="256x128")
-    assert (p.tiles()[(M > 0) & (N > 0)] == 6).all()
-    _cover(p, M, N)
-    for prec in ("fp32", "fp16"):
+    assert bg.Plan(M, N, K, prec="fp16", host_only=True).packed_bytes > 0
+    assert not (bg.Plan(M, N, K, prec="fp16", host_only=True, pack_operands=-1).tiles() == 6).any()
+    for prec, o in (("bf16x3", {}), ("fp16", {})):
+        p = bg.Plan(M, N, K, prec=prec, host_only=True, policy="fixed", fixed_tile="256x128", **o)
+        assert (p.tiles()[(M > 0) & (N > 0)] == 6).all()
+        _cover(p, M, N)
+    for prec, o in (("fp32", {}), ("fp8", {}), ("fp16", {"pack_operands": -1})):
         with pytest.raises(bg.VbgError):
-            bg.Plan(M, N, K, prec=prec, host_only=True, policy="fixed", fixed_tile="256x128")
+            bg.Plan(M, N, K, prec=prec, host_only=True, policy="fixed", fixed_tile="256x128", **o)
